@@ -1,0 +1,310 @@
+"""Pearl per-step solver on MI355X: Python binding of the C ABI.
+
+The host-side mirror of the reference's step interface (pearl_serial.cpp):
+
+  Solver.advance(stop_time)          advance            :1119-1174
+  Solver.maybe_shift(...)            maybe_shift        :2053-2071
+  Solver.propose_phi(dt)             propose_phi        :961-1002
+  Solver.implicit_diffusion(phi, dt) implicit_diffusion :1010-1094
+  Solver.reduced_energy()            reduced_energy     :793-821
+  Solver.total_carbon()              total_carbon       :784-788
+  Solver.valid_state()               valid_state        :1096-1116
+  Solver.diagnostics()               snapshot reductions :2267-2282
+
+Errors raise PearlError carrying the reference's exception text.  Every call
+runs the HIP kernels of libpearl_gpu.so; importing the package on a machine
+without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libpearl_gpu.so")
+
+OK = 0
+ERR_INVALID_TIME, ERR_DT_MIN, ERR_RETRY_LIMIT, ERR_MIXTURE, ERR_PROJECTION = 1, 2, 3, 4, 5
+ERR_UNSAFE_SHIFT, ERR_NONFINITE_CARBON, ERR_INVALID_SHIFT, ERR_ALLOC, ERR_ARGUMENT, ERR_DEVICE = 6, 7, 8, 9, 10, 11
+WHY = ["phase increment", "linear/closure", "composition/phase bounds", "mass conservation",
+       "free-energy increase"]
+FP64 = 0
+
+KERNELS = ["propose_phi", "diffusion_setup", "cg_residual0", "cg_apply_dot", "cg_update_dots",
+           "cg_direction", "true_residual", "cg_restart", "finalize_accept", "state_reduce",
+           "moving_window"]
+
+_dp = C.POINTER(C.c_double)
+
+
+class PearlError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class Params(C.Structure):
+    """pearl_params (include/pearl_gpu.h)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+        ("A", C.c_double * 3), ("B", C.c_double * 3), ("C", C.c_double * 3), ("D", C.c_double * 3),
+        ("eps", C.c_double), ("sigma", C.c_double), ("triple", C.c_double),
+        ("tau", C.c_double * 3), ("Vm", C.c_double),
+        ("dt_max", C.c_double), ("dt_min", C.c_double), ("max_phase_change", C.c_double),
+        ("cg_rtol", C.c_double), ("cg_atol", C.c_double), ("closure_tol", C.c_double),
+        ("energy_rtol", C.c_double), ("energy_atol", C.c_double),
+        ("cg_max", C.c_int), ("max_retries", C.c_int), ("check_energy", C.c_int), ("phase_only", C.c_int),
+    ]
+
+
+class Scalars(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("dt", C.c_double), ("shift_distance", C.c_double),
+        ("initial_carbon", C.c_double), ("carbon_added", C.c_double), ("carbon_removed", C.c_double),
+        ("step", C.c_longlong), ("shifts", C.c_longlong), ("rejected", C.c_longlong),
+    ]
+    NAMES = ["time", "dt", "shift_distance", "initial_carbon", "carbon_added", "carbon_removed",
+             "step", "shifts", "rejected"]
+
+    @classmethod
+    def from_list(cls, v):
+        s = cls()
+        for k, x in zip(cls.NAMES, v):
+            setattr(s, k, int(x) if k in ("step", "shifts", "rejected") else float(x))
+        return s
+
+    def to_list(self):
+        return [float(getattr(self, k)) for k in self.NAMES]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [
+        ("accepted_dt", C.c_double), ("max_phase_change", C.c_double), ("linear_residual", C.c_double),
+        ("closure", C.c_double), ("energy_change", C.c_double),
+        ("cg_iterations", C.c_int), ("retries", C.c_int), ("why", C.c_int),
+    ]
+
+
+class DiffusionInfo(C.Structure):
+    _fields_ = [("ok", C.c_int), ("it", C.c_int), ("residual", C.c_double), ("closure", C.c_double)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("c_min", C.c_double), ("c_max", C.c_double), ("mu_min", C.c_double), ("mu_max", C.c_double),
+                ("phi_sum_error", C.c_double), ("mass", C.c_double), ("energy", C.c_double)]
+
+
+_lib = None
+
+# Every symbol declared in include/pearl_gpu.h.
+EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state", "pearl_get_state",
+           "pearl_advance", "pearl_run", "pearl_maybe_shift", "pearl_diagnostics", "pearl_propose_phi",
+           "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
+           "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters"]
+
+
+def lib():
+    """Load libpearl_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(hipcc, gfx950); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.pearl_create.argtypes = [C.POINTER(Params), C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.pearl_destroy.argtypes = [vp]
+        L.pearl_last_error.restype = C.c_char_p
+        L.pearl_last_error.argtypes = [vp]
+        L.pearl_set_state.argtypes = [vp, _dp, _dp, _dp, C.POINTER(Scalars)]
+        L.pearl_get_state.argtypes = [vp, _dp, _dp, _dp, C.POINTER(Scalars)]
+        L.pearl_advance.argtypes = [vp, C.c_double, C.POINTER(StepInfo)]
+        L.pearl_run.argtypes = [vp, C.c_double, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                C.POINTER(C.c_int), C.POINTER(StepInfo)]
+        L.pearl_maybe_shift.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
+        L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
+        L.pearl_propose_phi.argtypes = [vp, C.c_double, _dp, _dp]
+        L.pearl_implicit_diffusion.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.POINTER(DiffusionInfo)]
+        L.pearl_reduced_energy.argtypes = [vp, _dp]
+        L.pearl_total_carbon.argtypes = [vp, _dp]
+        L.pearl_valid_state.argtypes = [vp, C.POINTER(C.c_int)]
+        L.pearl_get_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.pearl_profile.argtypes = [vp, C.c_int]
+        L.pearl_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), _dp, _dp]
+        L.pearl_kernel_name.restype = C.c_char_p
+        L.pearl_kernel_name.argtypes = [C.c_int]
+        L.pearl_counters.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+PARAM_KEYS = ["A0", "B0", "C0", "D0", "A1", "B1", "C1", "D1", "A2", "B2", "C2", "D2",
+              "eps", "sigma", "triple", "tau_at", "tau_ag", "tau_tg", "Vm", "dx", "dy", "nx", "ny",
+              "dt_max", "dt_min", "max_phase_change", "cg_rtol", "cg_atol", "closure_tol",
+              "energy_rtol", "energy_atol", "cg_max", "max_retries", "check_energy", "phase_only"]
+
+
+def params_from_dict(d: dict) -> Params:
+    """Build pearl_params from a dict with the keys of PARAM_KEYS."""
+    p = Params()
+    p.nx, p.ny = int(d["nx"]), int(d["ny"])
+    p.dx, p.dy = float(d["dx"]), float(d["dy"])
+    for i in range(3):
+        p.A[i], p.B[i], p.C[i], p.D[i] = d[f"A{i}"], d[f"B{i}"], d[f"C{i}"], d[f"D{i}"]
+    p.eps, p.sigma, p.triple = d["eps"], d["sigma"], d["triple"]
+    p.tau[0], p.tau[1], p.tau[2] = d["tau_at"], d["tau_ag"], d["tau_tg"]
+    p.Vm = d["Vm"]
+    for k in ("dt_max", "dt_min", "max_phase_change", "cg_rtol", "cg_atol", "closure_tol",
+              "energy_rtol", "energy_atol"):
+        setattr(p, k, float(d[k]))
+    p.cg_max, p.max_retries = int(d["cg_max"]), int(d["max_retries"])
+    p.check_energy, p.phase_only = int(d["check_energy"]), int(d["phase_only"])
+    return p
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class Solver:
+    """One device-resident domain (a pearl_ctx)."""
+
+    def __init__(self, params, mu_reference: float, device: int = 0, cells_per_thread: int = 0,
+                 precision: int = FP64):
+        if isinstance(params, dict):
+            params = params_from_dict(params)
+        self.params = params
+        self.nx, self.ny = params.nx, params.ny
+        self.mu_reference = float(mu_reference)
+        h = C.c_void_p()
+        rc = lib().pearl_create(C.byref(params), self.mu_reference, precision, device, cells_per_thread,
+                                C.byref(h))
+        if rc != OK:
+            raise PearlError(rc, f"pearl_create failed with status {rc}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pearl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise PearlError(rc, lib().pearl_last_error(self._h).decode())
+
+    # state -----------------------------------------------------------------
+    def set_state(self, phi, carbon, mu, scalars=None):
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(3, self.ny, self.nx)
+        carbon = np.ascontiguousarray(carbon, dtype=np.float64).reshape(self.ny, self.nx)
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(self.ny, self.nx)
+        sc = None
+        if scalars is not None:
+            sc = scalars if isinstance(scalars, Scalars) else Scalars.from_list(scalars)
+        self._check(lib().pearl_set_state(self._h, _ptr(phi), _ptr(carbon), _ptr(mu),
+                                          C.byref(sc) if sc is not None else None))
+
+    def get_state(self):
+        phi = np.empty((3, self.ny, self.nx))
+        c = np.empty((self.ny, self.nx))
+        mu = np.empty((self.ny, self.nx))
+        sc = Scalars()
+        self._check(lib().pearl_get_state(self._h, _ptr(phi), _ptr(c), _ptr(mu), C.byref(sc)))
+        return phi, c, mu, sc
+
+    def scalars(self) -> Scalars:
+        sc = Scalars()
+        self._check(lib().pearl_get_state(self._h, None, None, None, C.byref(sc)))
+        return sc
+
+    # the step -----------------------------------------------------------------
+    def advance(self, stop_time: float) -> StepInfo:
+        info = StepInfo()
+        self._check(lib().pearl_advance(self._h, stop_time, C.byref(info)))
+        return info
+
+    def run(self, stop_time: float, max_steps: int):
+        taken, cg_total, cg_peak = C.c_longlong(0), C.c_longlong(0), C.c_int(0)
+        last = StepInfo()
+        self._check(lib().pearl_run(self._h, stop_time, max_steps, C.byref(taken), C.byref(cg_total),
+                                    C.byref(cg_peak), C.byref(last)))
+        return dict(taken=taken.value, cg_total=cg_total.value, cg_peak=cg_peak.value, last=last)
+
+    def maybe_shift(self, moving_window: bool, trigger: float, target: float, c_gamma: float):
+        sh = C.c_int(0)
+        fr = C.c_double(0)
+        self._check(lib().pearl_maybe_shift(self._h, int(bool(moving_window)), trigger, target, c_gamma,
+                                            C.byref(sh), C.byref(fr)))
+        return bool(sh.value), fr.value
+
+    def diagnostics(self) -> dict:
+        d = Diag()
+        self._check(lib().pearl_diagnostics(self._h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    # single reference functions ------------------------------------------------
+    def propose_phi(self, dt: float):
+        out = np.empty((3, self.ny, self.nx))
+        ch = C.c_double(0)
+        self._check(lib().pearl_propose_phi(self._h, dt, _ptr(out), C.byref(ch)))
+        return ch.value, out
+
+    def implicit_diffusion(self, phi_trial, dt: float):
+        phi_trial = np.ascontiguousarray(phi_trial, dtype=np.float64).reshape(3, self.ny, self.nx)
+        c = np.empty((self.ny, self.nx))
+        mu = np.empty((self.ny, self.nx))
+        info = DiffusionInfo()
+        self._check(lib().pearl_implicit_diffusion(self._h, _ptr(phi_trial), dt, _ptr(c), _ptr(mu),
+                                                   C.byref(info)))
+        return dict(ok=info.ok, it=info.it, residual=info.residual, closure=info.closure, carbon=c, mu=mu)
+
+    def reduced_energy(self) -> float:
+        e = C.c_double(0)
+        self._check(lib().pearl_reduced_energy(self._h, C.byref(e)))
+        return e.value
+
+    def total_carbon(self) -> float:
+        m = C.c_double(0)
+        self._check(lib().pearl_total_carbon(self._h, C.byref(m)))
+        return m.value
+
+    def valid_state(self) -> bool:
+        v = C.c_int(0)
+        self._check(lib().pearl_valid_state(self._h, C.byref(v)))
+        return bool(v.value)
+
+    # layout, timing ----------------------------------------------------------
+    def layout(self):
+        tpb, k = C.c_int(0), C.c_int(0)
+        self._check(lib().pearl_get_layout(self._h, C.byref(tpb), C.byref(k)))
+        return tpb.value, k.value
+
+    def profile(self, enable: bool):
+        self._check(lib().pearl_profile(self._h, int(bool(enable))))
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNELS):
+            n, ms, b = C.c_longlong(0), C.c_double(0), C.c_double(0)
+            self._check(lib().pearl_kernel_stats(self._h, k, C.byref(n), C.byref(ms), C.byref(b)))
+            out[name] = dict(launches=n.value, ms=ms.value, bytes=b.value)
+        return out
+
+    def counters(self):
+        a, r, cg = C.c_longlong(0), C.c_longlong(0), C.c_longlong(0)
+        self._check(lib().pearl_counters(self._h, C.byref(a), C.byref(r), C.byref(cg)))
+        return dict(attempts=a.value, rejected_attempts=r.value, cg_iterations=cg.value)

@@ -1,0 +1,105 @@
+"""Input-file generator for the lamellar-colony workloads.
+
+Writes Pearl key=value input files (the format parsed by read_config,
+pearl_serial.cpp:1600-1758) with the physical parameters of the reference's
+default phase-diagram input (examples/fe_c_phase_diagram_10K_smoke.in) and a
+FILLCUBE lamella recipe (pearl_serial.cpp:1399-1405) for the initial colony.
+"""
+from __future__ import annotations
+
+import math
+import random
+
+# Physical parameters of the default 10 K phase-diagram input.
+DEFAULT_PARAMETERS = [
+    ("thermo_mode", "eutectoid_reference"),
+    ("T_eutectoid", "999.68459774"),
+    ("undercooling", "10"),
+    ("temperature_law", "phase_diagram_fixed_curvature"),
+    ("max_undercooling", "10"),
+    ("Aeq_alpha", "4724862.6373624019"),
+    ("Beq_alpha", "22964.296428571855"),
+    ("Ceq_alpha", "-42255.03701292308"),
+    ("D_alpha", "2e-9"),
+    ("Aeq_theta", "100000000"),
+    ("Beq_theta", "-49968654.66102279"),
+    ("Ceq_theta", "6207741.2462954065"),
+    ("D_theta", "2e-9"),
+    ("Aeq_gamma", "172591.65731409981"),
+    ("Beq_gamma", "19450.725112218221"),
+    ("Ceq_gamma", "-42053.816371518253"),
+    ("D_gamma", "1e-9"),
+    ("ceq_alpha", "0.00088690859310575857"),
+    ("ceq_theta", "0.25"),
+    ("ceq_gamma", "0.034458832049305582"),
+    ("slope_alpha_on_alpha_gamma", "-4.6282639461434345e-06"),
+    ("slope_gamma_on_alpha_gamma", "-0.00031198492411304955"),
+    ("slope_theta_on_theta_gamma", "0"),
+    ("slope_gamma_on_theta_gamma", "0.00010753713136114516"),
+    ("dx", "2.79e-9"),
+    ("Vm", "1e-6"),
+    ("epsilon", "14e-9"),
+    ("sigma", "0.5"),
+    ("triple", "10"),
+    ("tau_at", "1.72402799784e8"),
+    ("tau_ag", "1.72402799784e8"),
+    ("tau_tg", "7.11828875478e9"),
+    ("dt", "1.5e-11"),
+    ("dt_max", "1.5e-11"),
+    ("dt_min", "1e-16"),
+    ("gamma_carbon", "0.034458832049305582"),
+]
+
+
+def lamella_fills(nx: int, pairs: int, strip_rows: int, theta_fraction: float = 1.0 / 8.0,
+                  spacing_noise: float = 0.0, seed: int = 0):
+    """FILLCUBE lines for `pairs` cementite+ferrite lamella pairs across the
+    periodic width, in rows 0..strip_rows-1.  Each pair starts with its
+    cementite plate.  With spacing_noise>0 the pair periods are drawn as
+    f_i ~ U[1-noise, 1+noise] (seeded), normalised to fill nx exactly."""
+    rng = random.Random(seed)
+    if spacing_noise > 0:
+        f = [rng.uniform(1 - spacing_noise, 1 + spacing_noise) for _ in range(pairs)]
+    else:
+        f = [1.0] * pairs
+    tot = sum(f)
+    widths = [int(round(nx * fi / tot)) for fi in f]
+    widths[-1] += nx - sum(widths)
+    fills = []
+    x0 = 0
+    for w in widths:
+        th = max(1, int(round(w * theta_fraction)))
+        fills.append(("FILLCUBE", f"{{1,{x0},0,0,{x0 + th - 1},{strip_rows - 1},0}}"))
+        fills.append(("FILLCUBE", f"{{0,{x0 + th},0,0,{x0 + w - 1},{strip_rows - 1},0}}"))
+        x0 += w
+    return fills
+
+
+def write_input(path: str, nx: int, ny: int, end_time: float, output_dt: float, fills,
+                overrides: dict | None = None, extra: dict | None = None) -> str:
+    """Write an input file with the default physics, nx*ny cells, the given
+    FILL recipe (initialization=microsim) and optional key overrides."""
+    keys = dict(DEFAULT_PARAMETERS)
+    keys.update({"nx": str(nx), "ny": str(ny), "end_time": repr(float(end_time)),
+                 "output_dt": repr(float(output_dt)), "initialization": "microsim"})
+    for k, v in (overrides or {}).items():
+        keys[k] = str(v)
+    for k, v in (extra or {}).items():
+        keys[k] = str(v)
+    with open(path, "w") as f:
+        f.write("# generated lamellar-colony input\n")
+        for k, v in keys.items():
+            f.write(f"{k} = {v}\n")
+        for k, v in fills:
+            f.write(f"{k} = {v}\n")
+    return path
+
+
+def colony_input(path: str, nx: int, ny: int, pairs: int, strip_rows: int, end_time: float,
+                 output_dt: float | None = None, undercooling: float = 10.0, spacing_noise: float = 0.0,
+                 seed: int = 0, overrides: dict | None = None) -> str:
+    fills = lamella_fills(nx, pairs, strip_rows, spacing_noise=spacing_noise, seed=seed)
+    ov = {"undercooling": repr(float(undercooling)),
+          "max_undercooling": repr(float(max(10.0, math.ceil(undercooling))))}
+    ov.update(overrides or {})
+    return write_input(path, nx, ny, end_time, output_dt if output_dt else end_time * 10, fills, ov)

@@ -1,0 +1,176 @@
+/* pearl_gpu.h — C ABI of the MI355X (gfx950) per-step solver for Pearl.
+ *
+ * The reference has no plugin/FFI layer; its seam is the C++ call
+ *   StepInfo advance(const Domain&, State&, const Thermo&, const PhaseParameters&,
+ *                    const Controls&, double mu_reference, double stop_time)
+ *   (pearl_serial.cpp:1119, called only from the step loop :2303)
+ * plus maybe_shift (:2053, called at output times :2311).  This header
+ * replaces that seam: one opaque context owns the device-resident State, and
+ * each entry point below replaces the reference function named beside it.
+ *
+ * Conventions
+ *  - Plain pointers and sizes; no C++ or torch types.  Host arrays are
+ *    caller-owned and copied in/out (no aliasing).
+ *  - Fields are row-major [ny][nx] planes (the layout of the HDF5 datasets,
+ *    pearl_serial.cpp:1895-1929); phi is three planes alpha, theta, gamma.
+ *  - Every call is synchronous with respect to the state; one context per
+ *    host thread.  Return value: PEARL_OK (0) or a PEARL_ERR_* code; the
+ *    message of the reference's exception is available from
+ *    pearl_last_error().
+ *  - The FP64 plain form computes every per-cell expression in the
+ *    reference's order and rounding; global sums use a fixed blocked order
+ *    (pearl_get_layout reports it), max/min/AND reductions are exact.
+ */
+#ifndef PEARL_GPU_H
+#define PEARL_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Messages are the reference's exception texts. */
+#define PEARL_OK 0
+#define PEARL_ERR_INVALID_TIME 1     /* "invalid advance time"                       :1121 */
+#define PEARL_ERR_DT_MIN 2           /* "step rejected below dt_min: " + why          :1171 */
+#define PEARL_ERR_RETRY_LIMIT 3      /* "step retry limit: " + why                    :1173 */
+#define PEARL_ERR_MIXTURE 4          /* "Invalid mixture coefficients"                :136  */
+#define PEARL_ERR_PROJECTION 5       /* "Simplex projection failed"                   :505  */
+#define PEARL_ERR_UNSAFE_SHIFT 6     /* "unsafe shift: removed strip contains gamma"  :2068 */
+#define PEARL_ERR_NONFINITE_CARBON 7 /* "Nonfinite total carbon"                      :143  */
+#define PEARL_ERR_INVALID_SHIFT 8    /* "invalid shift"                               :2024 */
+#define PEARL_ERR_ALLOC 9            /* device allocation failed */
+#define PEARL_ERR_ARGUMENT 10        /* invalid argument (sizes, pointers, parameters) */
+#define PEARL_ERR_DEVICE 11          /* HIP runtime error */
+
+/* Rejection reasons, the reference's "why" strings (:1132-1160). */
+#define PEARL_WHY_PHASE 0   /* "phase increment" */
+#define PEARL_WHY_LINEAR 1  /* "linear/closure" */
+#define PEARL_WHY_BOUNDS 2  /* "composition/phase bounds" */
+#define PEARL_WHY_MASS 3    /* "mass conservation" */
+#define PEARL_WHY_ENERGY 4  /* "free-energy increase" */
+
+/* Precision forms. */
+#define PEARL_FP64 0        /* plain form: FP64 throughout */
+
+/* Model and numerical parameters: the reference's Domain (:617), Thermo
+ * (:76-91), PhaseParameters (:521-532) and Controls (:767-773). */
+typedef struct {
+    int nx, ny;                        /* global cells; x periodic, y zero flux */
+    double dx, dy;                     /* m */
+    double A[3], B[3], C[3], D[3];     /* f_p(c)=A c^2+B c+C [J/mol]; D_p [m^2/s] */
+    double eps, sigma, triple;         /* interface parameters */
+    double tau[3];                     /* tau_at, tau_ag, tau_tg [J s/m^4] */
+    double Vm;                         /* m^3/mol */
+    double dt_max, dt_min, max_phase_change;
+    double cg_rtol, cg_atol, closure_tol;
+    double energy_rtol, energy_atol;
+    int cg_max, max_retries, check_energy, phase_only;
+} pearl_params;
+
+/* State scalars (State, :758-764). */
+typedef struct {
+    double time, dt, shift_distance, initial_carbon, carbon_added, carbon_removed;
+    long long step, shifts, rejected;
+} pearl_scalars;
+
+/* StepInfo (:774-777) plus the reason of the last rejection. */
+typedef struct {
+    double accepted_dt, max_phase_change, linear_residual, closure, energy_change;
+    int cg_iterations, retries, why;
+} pearl_step_info;
+
+/* DiffusionInfo (:1003-1007). */
+typedef struct {
+    int ok, it;
+    double residual, closure;
+} pearl_diffusion_info;
+
+/* Output-time diagnostics of the reference's snapshot (:2266-2292). */
+typedef struct {
+    double c_min, c_max, mu_min, mu_max, phi_sum_error, mass, energy;
+} pearl_diag;
+
+typedef struct pearl_ctx pearl_ctx;
+
+/* Create a context on HIP device `device` for one domain.  mu_reference is
+ * initialize()'s return value (:1761, :2231).  cells_per_thread selects the
+ * blocked reduction layout (0 = automatic).  Replaces the construction of
+ * Domain/State (:2209, :2226). */
+int pearl_create(const pearl_params* params, double mu_reference, int precision, int device,
+                 int cells_per_thread, pearl_ctx** out);
+int pearl_destroy(pearl_ctx* ctx);
+const char* pearl_last_error(const pearl_ctx* ctx);
+
+/* Upload / download the State (replaces load_global :725 and gather :695). */
+int pearl_set_state(pearl_ctx* ctx, const double* phi, const double* carbon, const double* mu,
+                    const pearl_scalars* scalars);
+int pearl_get_state(pearl_ctx* ctx, double* phi, double* carbon, double* mu, pearl_scalars* scalars);
+
+/* One accepted step (replaces advance, :1119-1174). */
+int pearl_advance(pearl_ctx* ctx, double stop_time, pearl_step_info* info);
+
+/* Up to max_steps accepted steps toward stop_time without host I/O between
+ * them (the step loop :2296-2305 between two output times).  Stops early at
+ * stop_time.  *taken receives the number of accepted steps; cg_total and
+ * cg_peak accumulate as in the reference main loop. */
+int pearl_run(pearl_ctx* ctx, double stop_time, long long max_steps, long long* taken,
+              long long* cg_total, int* cg_peak, pearl_step_info* last);
+
+/* Moving window at an output time (replaces maybe_shift :2053-2071 and
+ * shift_rows :2023-2052; close_mu :789).  front receives the detected front
+ * position (-1 if none). */
+int pearl_maybe_shift(pearl_ctx* ctx, int moving_window, double shift_trigger, double shift_target,
+                      double shift_gamma_carbon, int* shifted, double* front);
+
+/* Snapshot reductions (:2267-2282). */
+int pearl_diagnostics(pearl_ctx* ctx, pearl_diag* out);
+
+/* Single reference functions on the current state, for parity tests.
+ *   pearl_propose_phi        propose_phi :961 (phi_out: 3 planes, may be NULL)
+ *   pearl_implicit_diffusion implicit_diffusion :1010 with a given trial phi
+ *   pearl_reduced_energy     reduced_energy :793
+ *   pearl_total_carbon       total_carbon :784
+ *   pearl_valid_state        valid_state :1096 */
+int pearl_propose_phi(pearl_ctx* ctx, double dt, double* phi_out, double* max_change);
+int pearl_implicit_diffusion(pearl_ctx* ctx, const double* phi_trial, double dt, double* carbon_out,
+                             double* mu_out, pearl_diffusion_info* info);
+int pearl_reduced_energy(pearl_ctx* ctx, double* energy);
+int pearl_total_carbon(pearl_ctx* ctx, double* mass);
+int pearl_valid_state(pearl_ctx* ctx, int* valid);
+
+/* Blocked-reduction layout in use: threads per block (256) and cells per
+ * thread.  A sum over n cells is: block b owns cells [b*256*k, (b+1)*256*k);
+ * thread t adds cells b*256*k + j*256 + t for j = 0..k-1 in order; a 64-lane
+ * halving tree per wavefront; (w0+w1)+(w2+w3) per block; the block partials
+ * are reduced by one 256-thread block in the same way. */
+int pearl_get_layout(const pearl_ctx* ctx, int* threads_per_block, int* cells_per_thread);
+
+/* Per-kernel timing with HIP events on the context's stream.  Kernel ids:
+ * see PEARL_KERNEL_*.  When enabled, every launch is bracketed by events;
+ * pearl_kernel_stats returns the launch count, total milliseconds and the
+ * algorithmic HBM bytes of all launches of that kernel. */
+#define PEARL_KERNEL_PROPOSE 0
+#define PEARL_KERNEL_SETUP 1
+#define PEARL_KERNEL_RESIDUAL0 2
+#define PEARL_KERNEL_CG_AP 3
+#define PEARL_KERNEL_CG_UPDATE 4
+#define PEARL_KERNEL_CG_DIRECTION 5
+#define PEARL_KERNEL_TRUE_RESIDUAL 6
+#define PEARL_KERNEL_CG_RESTART 7
+#define PEARL_KERNEL_FINALIZE 8
+#define PEARL_KERNEL_STATE_REDUCE 9
+#define PEARL_KERNEL_WINDOW 10
+#define PEARL_KERNEL_COUNT 11
+int pearl_profile(pearl_ctx* ctx, int enable);
+int pearl_kernel_stats(pearl_ctx* ctx, int kernel, long long* launches, double* total_ms, double* bytes);
+const char* pearl_kernel_name(int kernel);
+
+/* Attempt/iteration counters since creation: attempts, rejected attempts,
+ * total CG iterations over all attempts. */
+int pearl_counters(const pearl_ctx* ctx, long long* attempts, long long* rejected_attempts,
+                   long long* cg_iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

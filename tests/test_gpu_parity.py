@@ -1,0 +1,183 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+reference's own outputs (golden fixtures).
+
+Bars:
+ - per-cell work (propose_phi, max reductions, acceptance flags): bitwise
+   equal to the reference;
+ - everything: bitwise equal to the oracle run in the HIP path's blocked
+   reduction order (same cells-per-thread layout);
+ - against the reference's sequential sums: within the FP64 tolerance of the
+   target (|dmu| <= 1e-5 J/mol, |dphi| <= 1e-10), with equal step counts,
+   rejections and CG iterations on these fixtures.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["small_48x40", "phase_only_48x40", "window_alpha_32x40", "colony_256x128"]
+MU_TOL = 1e-5
+PHI_TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(pkg, g, k=0):
+    s = pkg.Solver(g["pd"], float(g["sc0"][9]), cells_per_thread=k)
+    s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+    return s
+
+
+def blocked_params(O, g, k):
+    return O.params_from_dict(g["pd"], red_mode=O.RED_BLOCKED, red_k=k)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("k", [1, 4])
+def test_propose_phi_bitwise_vs_reference(pkg, name, k):
+    g = load_golden(name)
+    say("propose_phi", name, "k", k)
+    with make(pkg, g, k) as s:
+        ch, phit = s.propose_phi(float(g["dt_probe"]))
+    assert ch == g["propose_change"]
+    assert np.array_equal(phit, g["propose_phi"])
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("k", [1, 2])
+def test_implicit_diffusion(pkg, O, name, k):
+    g = load_golden(name)
+    say("implicit_diffusion", name, "k", k)
+    with make(pkg, g, k) as s:
+        d = s.implicit_diffusion(g["propose_phi"], float(g["dt_probe"]))
+    P = blocked_params(O, g, k)
+    o = O.implicit_diffusion(P, g["propose_phi"], g["c0"], g["mu0"], float(g["dt_probe"]), float(g["sc0"][9]))
+    # bitwise against the oracle in the same reduction order
+    assert (d["ok"], d["it"]) == (o["ok"], o["it"])
+    assert d["residual"] == o["residual"] and d["closure"] == o["closure"]
+    assert np.array_equal(d["carbon"], o["carbon"]) and np.array_equal(d["mu"], o["mu"])
+    # against the reference's sequential order
+    assert (d["ok"], d["it"]) == (int(g["diff_ok"]), int(g["diff_it"]))
+    if d["ok"]:
+        assert np.abs(d["mu"] - g["diff_mu"]).max() <= MU_TOL
+        assert np.abs(d["carbon"] - g["diff_carbon"]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_state_reductions(pkg, O, name):
+    g = load_golden(name)
+    say("state_reductions", name)
+    with make(pkg, g) as s:
+        k = s.layout()[1]
+        e, m, v = s.reduced_energy(), s.total_carbon(), s.valid_state()
+    P = blocked_params(O, g, k)
+    eo, st = O.reduced_energy(P, g["phi0"], g["c0"], float(g["sc0"][9]))
+    assert e == eo
+    assert m == O.total_carbon(P, g["c0"])
+    assert v == bool(g["valid0"])
+    assert abs(e - float(g["energy0"])) <= 1e-13 * abs(float(g["energy0"]))
+    assert abs(m - float(g["carbon0"])) <= 1e-13 * abs(float(g["carbon0"]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_trajectory(pkg, O, name):
+    g = load_golden(name)
+    steps, end = int(g["steps"]), float(g["end_time"])
+    say("trajectory", name, steps)
+    with make(pkg, g) as s:
+        k = s.layout()[1]
+        infos = [s.advance(end) for _ in range(steps)]
+        phi, c, mu, sc = s.get_state()
+    # bitwise against the oracle in the blocked order
+    P = blocked_params(O, g, k)
+    S = O.OracleState(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+    for kk in range(steps):
+        rc, oi = O.advance(P, S, float(g["sc0"][9]), end)
+        assert rc == 0
+        gi = infos[kk]
+        assert [gi.accepted_dt, gi.max_phase_change, gi.linear_residual, gi.closure, gi.energy_change,
+                gi.cg_iterations, gi.retries] == [oi.accepted_dt, oi.max_phase_change, oi.linear_residual,
+                                                  oi.closure, oi.energy_change, oi.cg_iterations, oi.retries], kk
+    assert np.array_equal(phi, S.phi) and np.array_equal(c, S.carbon) and np.array_equal(mu, S.mu)
+    assert sc.to_list() == S.scalars()
+    # against the reference
+    ref = g["step_infos"]
+    assert [i.retries for i in infos] == [int(r[6]) for r in ref]
+    assert [i.cg_iterations for i in infos] == [int(r[5]) for r in ref]
+    assert [i.accepted_dt for i in infos] == [r[0] for r in ref]
+    assert np.abs(phi - g["phi1"]).max() <= PHI_TOL
+    assert np.abs(mu - g["mu1"]).max() <= MU_TOL
+    assert sc.to_list()[6:] == list(g["sc1"][6:9])
+
+
+@pytest.mark.parametrize("name", ["window_32x40", "window_alpha_32x40"])
+def test_moving_window(pkg, name):
+    g = load_golden(name)
+    say("moving_window", name)
+    with pkg.Solver(g["pd"], float(g["sc0"][9])) as s:
+        s.set_state(g["phi1"], g["c1"], g["mu1"], g["sc1"][:9])
+        err = str(g["shift_error"])
+        if err:
+            with pytest.raises(pkg.PearlError) as ei:
+                s.maybe_shift(True, *[float(v) for v in g["window"]])
+            assert str(ei.value) == err
+            return
+        shifted, front = s.maybe_shift(True, *[float(v) for v in g["window"]])
+        phi, c, mu, sc = s.get_state()
+    assert shifted
+    assert np.array_equal(phi, g["phi2"]) and np.array_equal(c, g["c2"])
+    assert np.array_equal(mu, g["mu2"])
+    ref = list(g["sc2"][:9])
+    got = sc.to_list()
+    assert got[:2] == ref[:2] and got[2] == ref[2] and got[4] == ref[4] and got[6:] == ref[6:]
+    assert abs(got[5] - ref[5]) <= 1e-13 * abs(ref[5])
+
+
+def test_diagnostics(pkg, O):
+    g = load_golden("small_48x40")
+    say("diagnostics")
+    with make(pkg, g) as s:
+        d = s.diagnostics()
+        k = s.layout()[1]
+    P = blocked_params(O, g, k)
+    assert d["c_min"] == g["c0"].min() and d["c_max"] == g["c0"].max()
+    assert d["mu_min"] == g["mu0"].min() and d["mu_max"] == g["mu0"].max()
+    ph = g["phi0"]
+    assert d["phi_sum_error"] == np.abs(ph[0] + ph[1] + ph[2] - 1).max()
+    assert d["mass"] == O.total_carbon(P, g["c0"])
+    assert d["energy"] == O.reduced_energy(P, g["phi0"], g["c0"], float(g["sc0"][9]))[0]
+
+
+def test_forced_rejection_paths(pkg):
+    """Host-forced failures: a tiny max_phase_change rejects every attempt
+    on the phase increment until dt falls below dt_min; cg_max=0 rejects on
+    linear/closure (the reference's messages, pearl_serial.cpp:1171)."""
+    g = load_golden("small_48x40")
+    say("forced_rejections")
+    pd = dict(g["pd"])
+    pd["max_phase_change"] = 1e-12
+    with pkg.Solver(pd, float(g["sc0"][9])) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        with pytest.raises(pkg.PearlError) as ei:
+            s.advance(float(g["end_time"]))
+        assert ei.value.code == pkg.ERR_DT_MIN
+        assert str(ei.value) == "step rejected below dt_min: phase increment"
+    pd = dict(g["pd"])
+    pd["cg_max"] = 0
+    with pkg.Solver(pd, float(g["sc0"][9])) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        with pytest.raises(pkg.PearlError) as ei:
+            s.advance(float(g["end_time"]))
+        assert str(ei.value) in ("step rejected below dt_min: linear/closure", "step retry limit: linear/closure")
+    with pkg.Solver(g["pd"], float(g["sc0"][9])) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        with pytest.raises(pkg.PearlError) as ei:
+            s.advance(0.0)
+        assert str(ei.value) == "invalid advance time"
