@@ -299,7 +299,6 @@ int advance_impl(pearl_ctx* c, double stop_time, pearl_step_info* info) {
     }
     const double e0 = prm.check_energy ? c->e0 : 0;
     const double mass0 = prm.phase_only ? 0 : c->m0;
-    const double dxdy = c->kp.dx * c->kp.dy;
     int why = PEARL_WHY_PHASE;
     for (int attempt = 0; attempt <= prm.max_retries; ++attempt) {
         c->attempts++;
@@ -334,9 +333,10 @@ int advance_impl(pearl_ctx* c, double stop_time, pearl_step_info* info) {
                 out.closure = d.closure;
                 why = PEARL_WHY_LINEAR;
             }
+            // (sum*dx)*dy as total_carbon :787 and reduced_energy :820
             valid = c->hS->valid;
-            mass = c->hS->mass * dxdy;
-            energy = c->hS->energy * dxdy;
+            mass = c->hS->mass * c->kp.dx * c->kp.dy;
+            energy = c->hS->energy * c->kp.dx * c->kp.dy;
         }
         if (good) {
             good = valid > 0.5;

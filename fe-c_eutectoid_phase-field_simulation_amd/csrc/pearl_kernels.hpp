@@ -270,7 +270,7 @@ __device__ __forceinline__ void accept_cell(const KP& P, const double* __restric
     for (int a = 0; a < 3; ++a) chem += h[a] * P.psiref[a];
     if (!P.phase_only) {
         Mix m;
-        mix_ok = mixture(P, p, m);
+        if (!mixture(P, p, m)) mix_ok = false;
         const double dc = c - composition(P.mu_ref, m);
         chem += .5 * dc * dc / m.s;
     }

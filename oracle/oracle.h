@@ -90,6 +90,9 @@ typedef struct {
     int why;                                 /* reason of the last rejection */
 } orc_stepinfo;
 
+/* project_simplex :491-509 (0 on success). */
+int orc_project_simplex(const double* trial, double* out);
+
 /* Sum in the configured order. */
 double orc_sum(const orc_params* P, const double* v, size_t n);
 

@@ -69,6 +69,8 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         P = C.POINTER(Params)
+        L.orc_project_simplex.restype = C.c_int
+        L.orc_project_simplex.argtypes = [_dp, _dp]
         L.orc_sum.restype = C.c_double
         L.orc_sum.argtypes = [P, _dp, C.c_size_t]
         L.orc_propose_phi.restype = C.c_double
@@ -184,6 +186,13 @@ def reduced_energy(P, phi, c, mu_ref):
     st = C.c_int(0)
     e = lib().orc_reduced_energy(C.byref(P), _ptr(phi), _ptr(c), mu_ref, C.byref(st))
     return e, st.value
+
+
+def project_simplex(z):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    out = np.zeros(3)
+    rc = lib().orc_project_simplex(_ptr(z), _ptr(out))
+    return rc, out
 
 
 def osum(P, v):

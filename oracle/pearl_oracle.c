@@ -145,6 +145,9 @@ static int project_simplex(const double* trial, double* out) {
     return ORC_OK;
 }
 
+/* Exported for the brute-force pin of the projection. */
+int orc_project_simplex(const double* trial, double* out) { return project_simplex(trial, out); }
+
 static inline void cell_phi(const double* phi, size_t n, size_t i, double* p) {
     p[0] = phi[i];
     p[1] = phi[n + i];

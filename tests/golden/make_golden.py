@@ -55,6 +55,21 @@ def case(name, path, steps, end_time, dt_probe, window=None, controls=None):
     print(name, "written:", steps, "steps, t =", sc1[0], "rejected =", sc1[8])
 
 
+def full_run(name, path, end_time):
+    """The whole trajectory to end_time with the reference main loop
+    (pearl_serial.cpp:2296-2316): initial state, final state and statistics."""
+    R = O.Ref(path)
+    R.initialize()
+    phi, c, mu, sc = R.get_state()
+    st = R.run(end_time, 10 ** 12)
+    phi1, c1, mu1, sc1 = R.get_state()
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), params=R.pvec, param_keys=np.array(O.PARAM_KEYS),
+                        phi0=phi, c0=c, mu0=mu, sc0=sc, end_time=end_time, output_dt=R.pd["output_dt"],
+                        taken=st["taken"], cg_total=st["cg_total"], cg_peak=st["cg_peak"],
+                        phi1=phi1, c1=c1, mu1=mu1, sc1=sc1)
+    print(name, "written:", st, "t =", sc1[0], "rejected =", sc1[8])
+
+
 def main():
     tmp = tempfile.mkdtemp()
     # Small colony: 2 lamella pairs, 20 smoothing steps, 40 steps.
@@ -80,6 +95,9 @@ def main():
     # rows 0-15, full 100-step smoothing; 20 steps.
     p = inputs.colony_input(os.path.join(tmp, "cfg0.in"), 256, 128, 4, 16, 1.5e-8, output_dt=2.5e-8)
     case("colony_256x128", p, 20, 1.5e-8, 1.5e-11)
+    if "--full" in sys.argv:
+        # Whole smoke-case trajectory to 0.015 us (about a minute of CPU).
+        full_run("colony_256x128_full", p, 1.5e-8)
 
 
 if __name__ == "__main__":

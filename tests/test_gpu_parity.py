@@ -82,8 +82,10 @@ def test_state_reductions(pkg, O, name):
     assert e == eo
     assert m == O.total_carbon(P, g["c0"])
     assert v == bool(g["valid0"])
-    assert abs(e - float(g["energy0"])) <= 1e-13 * abs(float(g["energy0"]))
-    assert abs(m - float(g["carbon0"])) <= 1e-13 * abs(float(g["carbon0"]))
+    # sequential vs blocked summation: bounded by n*eps relative (n <= 32768);
+    # the reference's own mass test allows 1e-11 relative (pearl_serial.cpp:1152)
+    assert abs(e - float(g["energy0"])) <= 1e-12 * abs(float(g["energy0"]))
+    assert abs(m - float(g["carbon0"])) <= 1e-12 * abs(float(g["carbon0"]))
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -137,7 +139,7 @@ def test_moving_window(pkg, name):
     ref = list(g["sc2"][:9])
     got = sc.to_list()
     assert got[:2] == ref[:2] and got[2] == ref[2] and got[4] == ref[4] and got[6:] == ref[6:]
-    assert abs(got[5] - ref[5]) <= 1e-13 * abs(ref[5])
+    assert abs(got[5] - ref[5]) <= 1e-12 * abs(ref[5])
 
 
 def test_diagnostics(pkg, O):
@@ -181,3 +183,27 @@ def test_forced_rejection_paths(pkg):
         with pytest.raises(pkg.PearlError) as ei:
             s.advance(0.0)
         assert str(ei.value) == "invalid advance time"
+
+
+def test_full_smoke_trajectory_vs_reference(pkg):
+    """The 256x128 lamellar smoke case to 0.015 us (the reference's own
+    horizon), from the reference's initialized state: identical accepted
+    steps, rejected attempts and PCG iteration total, and final fields
+    within the FP64 bar of the reference run."""
+    g = load_golden("colony_256x128_full")
+    end = float(g["end_time"])
+    say("full trajectory 256x128 to", end)
+    with pkg.Solver(g["pd"], float(g["sc0"][9])) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        r = s.run(end, 10 ** 9)
+        phi, c, mu, sc = s.get_state()
+    say("  steps", r["taken"], "cg", r["cg_total"], "rejected", sc.rejected)
+    assert r["taken"] == int(g["taken"])
+    assert r["cg_total"] == int(g["cg_total"]) and r["cg_peak"] == int(g["cg_peak"])
+    assert sc.rejected == int(g["sc1"][8]) and sc.step == int(g["sc1"][6])
+    assert abs(sc.time - float(g["sc1"][0])) <= 1e-12 * end
+    dphi = np.abs(phi - g["phi1"]).max()
+    dmu = np.abs(mu - g["mu1"]).max()
+    say("  max|dphi|", dphi, "max|dmu|", dmu, "max|dc|", np.abs(c - g["c1"]).max())
+    assert dphi <= PHI_TOL
+    assert dmu <= MU_TOL
