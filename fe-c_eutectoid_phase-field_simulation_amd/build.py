@@ -32,8 +32,39 @@ def sources():
     return [os.path.join(CSRC, "pearl_gpu.hip")]
 
 
+HDF5 = os.environ.get("PEARL_HDF5_PREFIX", "/opt/conda")
+EXE = os.path.join(PKG, "pearl_gpu")
+
+
+def build_host(verbose: bool = False) -> str | None:
+    """Compile the host program pearl_gpu (C++17) against libpearl_gpu.so and
+    the image's HDF5.  Returns None when no HDF5 is available."""
+    if not os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        return None
+    # The HIP runtime needs the system libstdc++, so the HDF5 library is
+    # reached through a private directory holding only a link to it (an rpath
+    # to the HDF5 prefix would also pick up that prefix's older libstdc++).
+    hdir = os.path.join(PKG, "_hdf5")
+    os.makedirs(hdir, exist_ok=True)
+    real = os.path.realpath(os.path.join(HDF5, "lib", "libhdf5.so"))
+    soname = "libhdf5.so.103"
+    link = os.path.join(hdir, soname)
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink(real, link)
+    tmp = EXE + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{HDF5}/include",
+           os.path.join(CSRC, "host", "pearl_host.cpp"), "-o", tmp, f"-L{PKG}", "-lpearl_gpu",
+           "-Wl,-rpath,$ORIGIN:$ORIGIN/_hdf5", link]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, EXE)
+    return EXE
+
+
 def build(verbose: bool = False) -> str:
-    """Compile libpearl_gpu.so (always recompiles)."""
+    """Compile libpearl_gpu.so and the host program (always recompiles)."""
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     tmp = LIB + ".tmp"
@@ -42,6 +73,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -25,16 +25,20 @@ const char* kWhy[5] = {"phase increment", "linear/closure", "composition/phase b
                        "free-energy increase"};
 
 const char* kKernelNames[PEARL_KERNEL_COUNT] = {
-    "propose_phi",  "diffusion_setup", "cg_residual0", "cg_apply_dot", "cg_update_dots", "cg_direction",
-    "true_residual", "cg_restart",     "finalize_accept", "state_reduce", "moving_window"};
+    "propose_phi", "diffusion_setup", "cg_residual0", "cg_apply", "cg_update", "true_residual",
+    "idle", "finalize_accept", "state_reduce", "moving_window"};
 
 // Algorithmic FP64 words per cell of each kernel (each logical array counted
 // once per pass): propose R phi(3),mu W phi'(3); setup R phi'(3),c,mu
-// W chi,icpt,K,rhs,x; residual0 R x,K,chi,rhs W r,diag,p; apply+dot R p,K,chi
-// W Ap; update R x,p,r,Ap,diag W x,r; direction R r,diag,p W p; true residual
-// R x,K,chi,rhs W r; restart R r,diag W p; finalize+accept R phi'(3),x,K,c,
-// icpt,chi W c',mu'; state reduce R phi(3),c,mu.
-const double kWords[PEARL_KERNEL_COUNT] = {7, 10, 7, 4, 7, 4, 5, 3, 10, 5, 0};
+// W chi,icpt,K,rhs,x; residual0 R x,K,chi,rhs W r,diag,z; apply R z,p_old,K,
+// chi W p,Ap; update R x,p,r,Ap,diag W x,r,z; true residual R x,K,chi,rhs,diag
+// W r,z; finalize+accept R phi'(3),x,K,c,icpt,chi W c',mu'; state reduce
+// R phi(3),c,mu.
+const double kWords[PEARL_KERNEL_COUNT] = {7, 10, 7, 6, 8, 7, 0, 10, 5, 0};
+
+// Launch accounting: a launch whose work depends on a device-side decision
+// is resolved after the next synchronisation.
+enum LaunchCond { COND_ALWAYS = 0, COND_LIVE = 1, COND_CG_OK = 2, COND_SLOT = 3 };
 
 }  // namespace
 
@@ -50,7 +54,8 @@ struct pearl_ctx {
     double *phi[2]{}, *c[2]{}, *mu[2]{};
     int cur = 0;
     double *chi = nullptr, *icpt = nullptr, *K = nullptr, *rhs = nullptr, *x = nullptr, *diag = nullptr;
-    double *r = nullptr, *p = nullptr, *Ap = nullptr, *part = nullptr;
+    double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *Ap = nullptr, *part = nullptr;
+    int cg_slots = 6;  // PCG slots enqueued per synchronisation
     DevScal* S = nullptr;
     DevScal* hS = nullptr;  // pinned host copy
     pearl_scalars sc{};
@@ -61,7 +66,8 @@ struct pearl_ctx {
     // profiling
     bool prof = false;
     struct Pending {
-        int kid;
+        int kid, cond, slot;
+        double cells;
         hipEvent_t a, b;
     };
     std::vector<Pending> pending;
@@ -99,34 +105,52 @@ hipEvent_t get_event(pearl_ctx* c) {
     return e;
 }
 
-// Fold completed event pairs into the per-kernel totals (stream is idle).
+// Resolve the launches since the last synchronisation (the stream is
+// idle): a launch counts under its kernel, with its algorithmic bytes, if it
+// did work, and under PEARL_KERNEL_CG_IDLE otherwise.
 void harvest(pearl_ctx* c) {
+    const DevScal* h = c->hS;
     for (auto& pd : c->pending) {
-        float t = 0;
-        (void)hipEventElapsedTime(&t, pd.a, pd.b);
-        c->ms[pd.kid] += t;
-        c->free_events.push_back(pd.a);
-        c->free_events.push_back(pd.b);
+        int kid = pd.kid;
+        bool real = true;
+        if (pd.cond == COND_LIVE) real = h->live != 0;
+        else if (pd.cond == COND_CG_OK) real = pd.slot == 1;
+        else if (pd.cond == COND_SLOT) {
+            const int kind = h->kind[pd.slot];
+            real = kind != KIND_NOOP;
+            if (kid == PEARL_KERNEL_CG_APPLY && kind == KIND_TRUE) kid = PEARL_KERNEL_TRUE_RESIDUAL;
+        }
+        if (!real) kid = PEARL_KERNEL_CG_IDLE;
+        c->launches[kid]++;
+        c->bytes[kid] += kWords[kid] * 8.0 * pd.cells;
+        if (pd.a) {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, pd.a, pd.b);
+            c->ms[kid] += t;
+            c->free_events.push_back(pd.a);
+            c->free_events.push_back(pd.b);
+        }
     }
     c->pending.clear();
 }
 
+// Brackets one launch: records it for accounting and, when profiling, with
+// a pair of events on the context's stream.
 struct ProfScope {
     pearl_ctx* c;
-    int kid;
-    hipEvent_t b{};
-    ProfScope(pearl_ctx* cc, int id, double cells) : c(cc), kid(id) {
-        c->launches[kid]++;
-        c->bytes[kid] += kWords[kid] * 8.0 * cells;
+    size_t idx;
+    ProfScope(pearl_ctx* cc, int kid, double cells, int cond = COND_ALWAYS, int slot = 0) : c(cc) {
+        pearl_ctx::Pending pd{kid, cond, slot, cells, nullptr, nullptr};
         if (c->prof) {
-            hipEvent_t a = get_event(c);
-            b = get_event(c);
-            (void)hipEventRecord(a, c->stream);
-            c->pending.push_back({kid, a, b});
+            pd.a = get_event(c);
+            pd.b = get_event(c);
+            (void)hipEventRecord(pd.a, c->stream);
         }
+        c->pending.push_back(pd);
+        idx = c->pending.size() - 1;
     }
     ~ProfScope() {
-        if (c->prof) (void)hipEventRecord(b, c->stream);
+        if (c->prof) (void)hipEventRecord(c->pending[idx].b, c->stream);
     }
 };
 
@@ -147,7 +171,7 @@ int sync_scalars(pearl_ctx* c) {
     HIP_CHECK(c, hipMemcpyAsync(c->hS, c->S, sizeof(DevScal), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     HIP_CHECK(c, hipGetLastError());
-    if (c->prof) harvest(c);
+    harvest(c);
     return PEARL_OK;
 }
 
@@ -173,22 +197,24 @@ int state_reduce(pearl_ctx* c, bool* valid, double* mass, double* energy, bool m
     }
     int st = sync_scalars(c);
     if (st) return st;
-    if (mixture_is_error && (c->hS->err & ERRF_MIXTURE)) return err_from_flags(c, ERRF_MIXTURE);
+    if (mixture_is_error && (c->hS->err & ERRF_MIXTURE_STATE)) return err_from_flags(c, ERRF_MIXTURE);
     if (c->hS->err) {
         c->hS->err = 0;
         hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(1), 0, c->stream, c->S);
     }
-    if (valid) *valid = c->hS->valid > 0.5;
-    if (mass) *mass = c->hS->mass * c->kp.dx * c->kp.dy;
-    if (energy) *energy = c->hS->energy * c->kp.dx * c->kp.dy;
+    if (valid) *valid = c->hS->e_valid > 0.5;
+    if (mass) *mass = c->hS->e_mass * c->kp.dx * c->kp.dy;
+    if (energy) *energy = c->hS->e_energy * c->kp.dx * c->kp.dy;
     return PEARL_OK;
 }
 
+void launch_propose(pearl_ctx* c, double dt) {
+    ProfScope ps(c, PEARL_KERNEL_PROPOSE, c->n);
+    PEARL_DISPATCH(c, k_propose, grid_of(c), c->kp, cur_phi(c), cur_mu(c), trial_phi(c), dt, c->part, c->S);
+}
+
 int propose(pearl_ctx* c, double dt, double* change) {
-    {
-        ProfScope ps(c, PEARL_KERNEL_PROPOSE, c->n);
-        PEARL_DISPATCH(c, k_propose, grid_of(c), c->kp, cur_phi(c), cur_mu(c), trial_phi(c), dt, c->part, c->S);
-    }
+    launch_propose(c, dt);
     int st = sync_scalars(c);
     if (st) return st;
     if (c->hS->err & ERRF_PROJECTION) return err_from_flags(c, ERRF_PROJECTION);
@@ -197,84 +223,99 @@ int propose(pearl_ctx* c, double dt, double* change) {
 }
 
 // implicit_diffusion :1010-1094 on (cur c, cur mu, trial phi) -> trial c, mu,
-// followed by the fused acceptance reductions of the trial state.
+// fused with the acceptance reductions of the trial state.  The PCG loop is
+// controlled on the device (k_residual0 / k_cg_a / k_cg_b finishers); the
+// host enqueues `cg_slots` iterations at a time, each predicated on the
+// device state, and synchronises once per batch.
 struct DiffOut {
     int ok = 0, it = 0;
     double residual = 0, closure = 0;
     bool finalized = false;
 };
 
-int diffusion(pearl_ctx* c, double dt, DiffOut& out) {
+void launch_setup(pearl_ctx* c, double dt) {
     const KP& P = c->kp;
     const dim3 g = grid_of(c);
     {
-        ProfScope ps(c, PEARL_KERNEL_SETUP, c->n);
+        ProfScope ps(c, PEARL_KERNEL_SETUP, c->n, COND_LIVE);
         PEARL_DISPATCH(c, k_setup, g, P, trial_phi(c), cur_c(c), cur_mu(c), c->chi, c->icpt, c->K, c->rhs, c->x, c->S);
     }
     {
-        ProfScope ps(c, PEARL_KERNEL_RESIDUAL0, c->n);
-        PEARL_DISPATCH(c, k_residual0, g, P, c->chi, c->K, c->rhs, c->x, c->diag, c->r, c->p, dt, c->part, c->S);
+        ProfScope ps(c, PEARL_KERNEL_RESIDUAL0, c->n, COND_LIVE);
+        PEARL_DISPATCH(c, k_residual0, g, P, c->chi, c->K, c->rhs, c->x, c->diag, c->r, c->z, dt, c->part, c->S);
     }
-    int st = sync_scalars(c);
-    if (st) return st;
-    if (c->hS->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
-    const double tolerance = c->prm.cg_atol + c->prm.cg_rtol * std::sqrt(c->hS->rhs2);
-    double rr = c->hS->rr;
-    bool ok = false;
-    for (int it = 0; it <= c->prm.cg_max; ++it) {
-        out.it = it;
-        out.residual = std::sqrt(rr);
-        if (!std::isfinite(out.residual)) return PEARL_OK;
-        if (out.residual <= tolerance) {
-            {
-                ProfScope ps(c, PEARL_KERNEL_TRUE_RESIDUAL, c->n);
-                PEARL_DISPATCH(c, k_true_residual, g, P, c->chi, c->K, c->rhs, c->x, c->r, dt, c->part, c->S);
-            }
-            st = sync_scalars(c);
-            if (st) return st;
-            rr = c->hS->rr;
-            out.residual = std::sqrt(rr);
-            if (out.residual <= tolerance * 1.05) {
-                ok = true;
-                break;
-            }
-            {
-                ProfScope ps(c, PEARL_KERNEL_CG_RESTART, c->n);
-                PEARL_DISPATCH(c, k_cg_restart, g, P, c->r, c->diag, c->p, c->part, c->S);
-            }
-            continue;
-        }
-        if (it == c->prm.cg_max) break;
+}
+
+// Enqueue `slots` PCG iterations and the (predicated) finalize.
+void launch_cg_batch(pearl_ctx* c, double dt, int slots) {
+    const KP& P = c->kp;
+    const dim3 g = grid_of(c);
+    for (int j = 0; j < slots; ++j) {
         {
-            ProfScope ps(c, PEARL_KERNEL_CG_AP, c->n);
-            PEARL_DISPATCH(c, k_cg_ap, g, P, c->chi, c->K, c->p, c->Ap, dt, c->part, c->S);
+            ProfScope ps(c, PEARL_KERNEL_CG_APPLY, c->n, COND_SLOT, 2 * j);
+            PEARL_DISPATCH(c, k_cg_a, g, P, c->chi, c->K, c->rhs, c->x, c->diag, c->r, c->z, c->p0, c->p1, c->Ap, dt,
+                           2 * j, c->part, c->S);
         }
         {
-            ProfScope ps(c, PEARL_KERNEL_CG_UPDATE, c->n);
-            PEARL_DISPATCH(c, k_cg_update, g, P, c->x, c->r, c->p, c->Ap, c->diag, c->part, c->S);
+            ProfScope ps(c, PEARL_KERNEL_CG_UPDATE, c->n, COND_SLOT, 2 * j + 1);
+            PEARL_DISPATCH(c, k_cg_b, g, P, c->x, c->r, c->z, c->p0, c->p1, c->Ap, c->diag, 2 * j + 1, c->part, c->S);
         }
-        {
-            ProfScope ps(c, PEARL_KERNEL_CG_DIRECTION, c->n);
-            PEARL_DISPATCH(c, k_cg_direction, g, P, c->r, c->diag, c->p, c->S);
-        }
-        st = sync_scalars(c);
+    }
+    ProfScope ps(c, PEARL_KERNEL_FINALIZE, c->n, COND_CG_OK, 0);
+    PEARL_DISPATCH(c, k_finalize, g, P, trial_phi(c), c->chi, c->icpt, c->K, c->x, cur_c(c), trial_c(c), trial_mu(c),
+                   dt, c->part, c->S);
+}
+
+// After the first synchronisation of an attempt: finish the PCG solve if
+// the enqueued iterations were not enough, and read its outcome.
+int sync_attempt(pearl_ctx* c);
+
+int finish_diffusion(pearl_ctx* c, double dt, DiffOut& out) {
+    while (c->hS->cg_state == CG_ITER || c->hS->cg_state == CG_TRUE) {
+        launch_cg_batch(c, dt, c->cg_slots);
+        const int st = sync_attempt(c);
         if (st) return st;
-        if (c->hS->bad) return PEARL_OK;
-        rr = c->hS->rr;
     }
-    c->cg_iterations += out.it;
-    if (!ok) return PEARL_OK;
-    {
-        ProfScope ps(c, PEARL_KERNEL_FINALIZE, c->n);
-        PEARL_DISPATCH(c, k_finalize, g, P, trial_phi(c), c->chi, c->icpt, c->K, c->x, cur_c(c), trial_c(c),
-                       trial_mu(c), dt, c->part, c->S);
-    }
-    st = sync_scalars(c);
-    if (st) return st;
-    out.closure = c->hS->closure;
+    const DevScal* h = c->hS;
+    out.it = h->cg_it;
+    out.residual = h->cg_res;
+    c->cg_iterations += h->cg_it;
+    c->cg_slots = std::min(28, std::max(2, h->cg_it + 2));
+    if (h->cg_state != CG_OK) return PEARL_OK;
+    out.closure = h->closure;
     out.ok = out.closure <= c->prm.closure_tol;
     out.finalized = true;
     return PEARL_OK;
+}
+
+// Marks the last finalize launch of the pending list as the one that ran.
+void mark_final_finalize(pearl_ctx* c) {
+    for (auto it = c->pending.rbegin(); it != c->pending.rend(); ++it)
+        if (it->cond == COND_CG_OK) {
+            it->slot = 1;
+            break;
+        }
+}
+
+// Synchronise after a batch.  The finalize launch runs only in the batch in
+// which the solve finished, and it is the last one enqueued in that batch.
+int sync_attempt(pearl_ctx* c) {
+    HIP_CHECK(c, hipMemcpyAsync(c->hS, c->S, sizeof(DevScal), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, hipGetLastError());
+    if (c->hS->cg_state == CG_OK) mark_final_finalize(c);
+    harvest(c);
+    return PEARL_OK;
+}
+
+int diffusion(pearl_ctx* c, double dt, DiffOut& out) {
+    hipLaunchKernelGGL(k_begin_solve, dim3(1), dim3(1), 0, c->stream, c->S);
+    launch_setup(c, dt);
+    launch_cg_batch(c, dt, c->cg_slots);
+    int st = sync_attempt(c);
+    if (st) return st;
+    if (c->hS->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
+    return finish_diffusion(c, dt, out);
 }
 
 int advance_impl(pearl_ctx* c, double stop_time, pearl_step_info* info) {
@@ -286,19 +327,7 @@ int advance_impl(pearl_ctx* c, double stop_time, pearl_step_info* info) {
     std::memset(info, 0, sizeof(*info));
     info->why = -1;
     if (!(dt > 0) || !std::isfinite(dt)) return fail(c, PEARL_ERR_INVALID_TIME, "invalid advance time");
-    if ((prm.check_energy && !c->e0_valid) || (!prm.phase_only && !c->m0_valid)) {
-        double m = 0, e = 0;
-        int st = state_reduce(c, nullptr, &m, &e, prm.check_energy != 0);
-        if (st) return st;
-        if (prm.check_energy) {
-            c->e0 = e;
-            c->e0_valid = true;
-        }
-        c->m0 = m;
-        c->m0_valid = true;
-    }
-    const double e0 = prm.check_energy ? c->e0 : 0;
-    const double mass0 = prm.phase_only ? 0 : c->m0;
+    bool need_state = (prm.check_energy && !c->e0_valid) || (!prm.phase_only && !c->m0_valid);
     int why = PEARL_WHY_PHASE;
     for (int attempt = 0; attempt <= prm.max_retries; ++attempt) {
         c->attempts++;
@@ -306,26 +335,48 @@ int advance_impl(pearl_ctx* c, double stop_time, pearl_step_info* info) {
         out.retries = attempt;
         out.accepted_dt = dt;
         out.why = -1;
-        double change = 0;
-        int st = propose(c, dt, &change);
+        // One batch: [state reductions] propose, setup, residual0, PCG
+        // iterations, finalize; everything after propose is predicated on
+        // the device-side phase-increment decision.
+        if (need_state) {
+            ProfScope ps(c, PEARL_KERNEL_STATE_REDUCE, c->n);
+            PEARL_DISPATCH(c, k_state_reduce, grid_of(c), c->kp, cur_phi(c), cur_c(c), cur_mu(c), c->part, c->S);
+        }
+        launch_propose(c, dt);
+        if (prm.phase_only) {
+            ProfScope ps(c, PEARL_KERNEL_FINALIZE, c->n, COND_LIVE);
+            PEARL_DISPATCH(c, k_phase_only, grid_of(c), c->kp, trial_phi(c), cur_mu(c), trial_c(c), trial_mu(c),
+                           c->part, c->S);
+        } else {
+            launch_setup(c, dt);
+            launch_cg_batch(c, dt, c->cg_slots);
+        }
+        int st = sync_attempt(c);
         if (st) return st;
+        const DevScal* h = c->hS;
+        if (need_state) {
+            if (prm.check_energy && (h->err & ERRF_MIXTURE_STATE)) return err_from_flags(c, ERRF_MIXTURE);
+            if (prm.check_energy) {
+                c->e0 = h->e_energy * c->kp.dx * c->kp.dy;
+                c->e0_valid = true;
+            }
+            c->m0 = h->e_mass * c->kp.dx * c->kp.dy;
+            c->m0_valid = true;
+            need_state = false;
+        }
+        if (h->err & ERRF_PROJECTION) return err_from_flags(c, ERRF_PROJECTION);
+        const double e0 = prm.check_energy ? c->e0 : 0;
+        const double mass0 = prm.phase_only ? 0 : c->m0;
+        const double change = h->change;
         out.max_phase_change = change;
         bool good = std::isfinite(change) && change <= prm.max_phase_change;
         why = PEARL_WHY_PHASE;
         double mass = 0, energy = 0, valid = 0;
         if (good) {
-            if (prm.phase_only) {
-                {
-                    ProfScope ps(c, PEARL_KERNEL_FINALIZE, c->n);
-                    PEARL_DISPATCH(c, k_phase_only, grid_of(c), c->kp, trial_phi(c), cur_mu(c), trial_c(c),
-                                   trial_mu(c), c->part, c->S);
-                }
-                st = sync_scalars(c);
-                if (st) return st;
-                if (c->hS->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
-            } else {
+            if (h->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
+            if (!prm.phase_only) {
                 DiffOut d;
-                st = diffusion(c, dt, d);
+                st = finish_diffusion(c, dt, d);
                 if (st) return st;
                 good = d.ok != 0;
                 out.cg_iterations = d.it;
@@ -457,12 +508,16 @@ int pearl_create(const pearl_params* params, double mu_reference, int precision,
     P.psiref[1] = psi(1, mu_reference) - psi(2, mu_reference);
     P.psiref[2] = 0;
     P.mu_ref = mu_reference;
+    P.max_phase_change = params->max_phase_change;
+    P.cg_rtol = params->cg_rtol;
+    P.cg_atol = params->cg_atol;
+    P.cg_max = params->cg_max;
     P.phase_only = params->phase_only;
 
     const size_t n = static_cast<size_t>(c->n);
     const size_t plane = (n + 31) & ~static_cast<size_t>(31);  // 256-byte aligned planes
     const size_t nparts = 8 * static_cast<size_t>(P.nb) + 64;
-    const size_t total = plane * 19 + nparts;
+    const size_t total = plane * 21 + nparts;
     if (hipMalloc(&c->mem, total * sizeof(double)) != hipSuccess) {
         delete c;
         return PEARL_ERR_ALLOC;
@@ -487,7 +542,9 @@ int pearl_create(const pearl_params* params, double mu_reference, int precision,
     c->x = q; q += plane;
     c->diag = q; q += plane;
     c->r = q; q += plane;
-    c->p = q; q += plane;
+    c->z = q; q += plane;
+    c->p0 = q; q += plane;
+    c->p1 = q; q += plane;
     c->Ap = q; q += plane;
     c->part = q;
     if (hipMalloc(&c->S, sizeof(DevScal)) != hipSuccess || hipHostMalloc(&c->hS, sizeof(DevScal)) != hipSuccess) {

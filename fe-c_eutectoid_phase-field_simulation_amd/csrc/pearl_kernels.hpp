@@ -32,6 +32,8 @@ struct KP {
     double tau[3], tau_min;
     double psiref[3];       // psi_a(mu_ref) - psi_gamma(mu_ref), psiref[2]=0  :797
     double mu_ref;
+    double max_phase_change, cg_rtol, cg_atol;  // controls used by device-side decisions
+    int cg_max;
     int phase_only;
 };
 
@@ -39,16 +41,42 @@ struct KP {
 struct DevScal {
     double change;                        // propose_phi max |dphi|
     double rhs2, rz, rr, pAp, alpha, beta;  // PCG
+    double tol, cg_res;                   // PCG tolerance (:1044) and info.residual
     double closure, valid, mass, energy;  // finalize + acceptance
+    double e_valid, e_mass, e_energy;     // state reductions of the current state
     double front, gmax, removed;          // moving window
     double dmin_c, dmax_c, dmin_mu, dmax_mu, dsumerr, dmass, denergy;  // diagnostics
-    int bad;                              // PCG breakdown (:1070)
+    int live;                             // attempt still running (phase increment passed)
+    int cg_state;                         // CG_* below
+    int cg_it;                            // info.it (:1048)
+    int restart;                          // next direction is p = z (first iteration, restart)
+    int pidx;                             // which p buffer holds the current direction
     int err;                              // ERRF_* bits
     unsigned counter;                     // last-block counter
     int pad;
+    int kind[64];                         // per-launch record of the PCG kernels (profiling)
 };
 
-enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4 };
+enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4, ERRF_MIXTURE_STATE = 8 };
+
+// PCG states (implicit_diffusion :1047-1080).  CG_ITER: take a step;
+// CG_TRUE: recompute the true residual (:1053); CG_OK / CG_FAIL: done;
+// CG_IDLE: no solve in this attempt.
+enum { CG_IDLE = 0, CG_ITER = 1, CG_TRUE = 2, CG_OK = 3, CG_FAIL = 4 };
+enum { KIND_NOOP = 0, KIND_ITER = 1, KIND_TRUE = 2 };
+
+// The loop head of implicit_diffusion (:1047-1067) for iteration `it` with
+// the current r.r: info.it, info.residual, then the convergence and
+// iteration-limit branches.  Runs in one thread.
+__device__ __forceinline__ void cg_head(const int cg_max, DevScal* S, int it) {
+    const double res = __builtin_sqrt(S->rr);
+    S->cg_it = it;
+    S->cg_res = res;
+    if (!__builtin_isfinite(res)) S->cg_state = CG_FAIL;
+    else if (res <= S->tol) S->cg_state = CG_TRUE;
+    else if (it == cg_max) S->cg_state = CG_FAIL;
+    else S->cg_state = CG_ITER;
+}
 
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }  // std::min
@@ -371,7 +399,12 @@ __global__ __launch_bounds__(TPB) void k_propose(KP P, const double* __restrict_
     for (int a = 0; a < 3; ++a) out[a * n + i] = zp[a];
     PEARL_CELL_LOOP_END
     if (proj_fail) atomicOr(&S->err, ERRF_PROJECTION);
-    if (block_reduce<1, 1u>(v, part, &S->counter, P.nb)) S->change = v[0];
+    if (block_reduce<1, 1u>(v, part, &S->counter, P.nb)) {
+        S->change = v[0];
+        // :1131 -- the rest of the attempt runs only if the increment passes
+        S->live = (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change) ? 1 : 0;
+        S->cg_state = CG_IDLE;
+    }
 }
 
 // P2: implicit_diffusion setup :1013-1020.
@@ -380,6 +413,7 @@ __global__ __launch_bounds__(TPB) void k_setup(KP P, const double* __restrict__ 
                                               const double* __restrict__ mu_old, double* __restrict__ chi,
                                               double* __restrict__ icpt, double* __restrict__ K,
                                               double* __restrict__ rhs, double* __restrict__ x, DevScal* S) {
+    if (!S->live) return;
     bool bad = false;
     PEARL_CELL_LOOP(KK)
     double p[3];
@@ -396,13 +430,15 @@ __global__ __launch_bounds__(TPB) void k_setup(KP P, const double* __restrict__ 
     if (bad) atomicOr(&S->err, ERRF_MIXTURE);
 }
 
-// P3: initial residual, Jacobi diagonal, first direction :1036-1045.
-// Sums: rhs.rhs, r.z, r.r.
+// P3: initial residual, Jacobi diagonal, preconditioned residual
+// (:1036-1045).  Sums rhs.rhs, r.z, r.r; the finisher forms the tolerance
+// and evaluates the loop head for iteration 0.
 template <int KK>
 __global__ __launch_bounds__(TPB) void k_residual0(KP P, const double* __restrict__ chi, const double* __restrict__ K,
                                                   const double* __restrict__ rhs, const double* __restrict__ x,
                                                   double* __restrict__ diag, double* __restrict__ r,
-                                                  double* __restrict__ p, double dt, double* part, DevScal* S) {
+                                                  double* __restrict__ z, double dt, double* part, DevScal* S) {
+    if (!S->live) return;
     double v[3] = {0.0, 0.0, 0.0};
     PEARL_CELL_LOOP(KK)
     const Nbr nb = neighbours(P, i);
@@ -413,59 +449,121 @@ __global__ __launch_bounds__(TPB) void k_residual0(KP P, const double* __restric
     const double d = c + dt * (w[0] + w[1] + w[2] + w[3]);
     const double b = rhs[i];
     const double rr = b - ax;
-    const double z = rr / d;
+    const double zz = rr / d;
     diag[i] = d;
     r[i] = rr;
-    p[i] = z;
+    z[i] = zz;
     v[0] += b * b;
-    v[1] += rr * z;
+    v[1] += rr * zz;
     v[2] += rr * rr;
     PEARL_CELL_LOOP_END
     if (block_reduce<3, 0u>(v, part, &S->counter, P.nb)) {
         S->rhs2 = v[0];
         S->rz = v[1];
         S->rr = v[2];
-        S->bad = 0;
+        S->tol = P.cg_atol + P.cg_rtol * __builtin_sqrt(v[0]);
+        S->restart = 1;
+        cg_head(P.cg_max, S, 0);
     }
 }
 
-// I1: Ap = A p and p.Ap; the finisher forms the step length (:1068-1071).
+// PCG kernel A.  In state CG_ITER: the search direction of this iteration
+// p = z + beta p_old (:1079; p = z after a (re)start :1042/:1062), kept in
+// the other p buffer, then Ap = A p (:1068) and p.Ap; the finisher checks
+// for breakdown (:1070) and forms the step length (:1071).  In state
+// CG_TRUE: the true residual r = rhs - A x with z = r/diag (:1053-1062);
+// the finisher accepts (:1056) or restarts from it (:1060-1065).
+// `slot` indexes the per-launch record used by the profiler.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_cg_ap(KP P, const double* __restrict__ chi, const double* __restrict__ K,
-                                              const double* __restrict__ p, double* __restrict__ Ap, double dt,
-                                              double* part, DevScal* S) {
-    double v[1] = {0.0};
-    PEARL_CELL_LOOP(KK)
-    const Nbr nb = neighbours(P, i);
-    double w[4];
-    face_weights(P, K, i, nb, w);
-    const double a = apply_cell(P, p, chi[i], dt, i, nb, w);
-    Ap[i] = a;
-    v[0] += p[i] * a;
-    PEARL_CELL_LOOP_END
-    if (block_reduce<1, 0u>(v, part, &S->counter, P.nb)) {
-        const double pAp = v[0], rz = S->rz;
-        S->pAp = pAp;
-        const bool bad = !(pAp > 0) || !__builtin_isfinite(pAp) || !__builtin_isfinite(rz);
-        S->bad = bad ? 1 : 0;
-        S->alpha = rz / pAp;
+__global__ __launch_bounds__(TPB) void k_cg_a(KP P, const double* __restrict__ chi, const double* __restrict__ K,
+                                             const double* __restrict__ rhs, const double* __restrict__ x,
+                                             const double* __restrict__ diag, double* __restrict__ r,
+                                             double* __restrict__ z, double* __restrict__ p0,
+                                             double* __restrict__ p1, double* __restrict__ Ap, double dt, int slot,
+                                             double* part, DevScal* S) {
+    const int state = S->cg_state;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && slot >= 0)
+        S->kind[slot] = state == CG_ITER ? KIND_ITER : (state == CG_TRUE ? KIND_TRUE : KIND_NOOP);
+    if (state == CG_ITER) {
+        const int pi = S->pidx;
+        const double* __restrict__ po = pi ? p1 : p0;
+        double* __restrict__ pn = pi ? p0 : p1;
+        const bool fresh = S->restart != 0;
+        const double beta = S->beta;
+        double v[1] = {0.0};
+        PEARL_CELL_LOOP(KK)
+        const Nbr nb = neighbours(P, i);
+        double w[4];
+        face_weights(P, K, i, nb, w);
+        const double pc = fresh ? z[i] : z[i] + beta * po[i];
+        const double pw = fresh ? z[nb.w] : z[nb.w] + beta * po[nb.w];
+        const double pe = fresh ? z[nb.e] : z[nb.e] + beta * po[nb.e];
+        const double ps = fresh ? z[nb.s] : z[nb.s] + beta * po[nb.s];
+        const double pnn = fresh ? z[nb.nn] : z[nb.nn] + beta * po[nb.nn];
+        const double a = chi[i] * pc + dt * (w[0] * (pc - pw) + w[1] * (pc - pe) + w[2] * (pc - ps) + w[3] * (pc - pnn));
+        pn[i] = pc;
+        Ap[i] = a;
+        v[0] += pc * a;
+        PEARL_CELL_LOOP_END
+        if (block_reduce<1, 0u>(v, part, &S->counter, P.nb)) {
+            const double pAp = v[0], rz = S->rz;
+            S->pAp = pAp;
+            if (!(pAp > 0) || !__builtin_isfinite(pAp) || !__builtin_isfinite(rz)) {
+                S->cg_state = CG_FAIL;
+            } else {
+                S->alpha = rz / pAp;
+                S->pidx = pi ^ 1;
+            }
+        }
+    } else if (state == CG_TRUE) {
+        double v[2] = {0.0, 0.0};
+        PEARL_CELL_LOOP(KK)
+        const Nbr nb = neighbours(P, i);
+        double w[4];
+        face_weights(P, K, i, nb, w);
+        const double rr = rhs[i] - apply_cell(P, x, chi[i], dt, i, nb, w);
+        const double zz = rr / diag[i];
+        r[i] = rr;
+        z[i] = zz;
+        v[0] += rr * rr;
+        v[1] += rr * zz;
+        PEARL_CELL_LOOP_END
+        if (block_reduce<2, 0u>(v, part, &S->counter, P.nb)) {
+            S->rr = v[0];
+            const double res = __builtin_sqrt(v[0]);
+            S->cg_res = res;
+            if (res <= S->tol * 1.05) {
+                S->cg_state = CG_OK;
+            } else {
+                S->rz = v[1];
+                S->restart = 1;
+                cg_head(P.cg_max, S, S->cg_it + 1);
+            }
+        }
     }
 }
 
-// I2: x += alpha p; r -= alpha Ap; z = r/diag; sums r.z, r.r (:1072-1078).
+// PCG kernel B (CG_ITER only): x += alpha p; r -= alpha Ap; z = r/diag; sums
+// r.z and r.r; the finisher forms beta (:1077) and evaluates the loop head
+// of the next iteration.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_cg_update(KP P, double* __restrict__ x, double* __restrict__ r,
-                                                  const double* __restrict__ p, const double* __restrict__ Ap,
-                                                  const double* __restrict__ diag, double* part, DevScal* S) {
-    if (S->bad) return;
+__global__ __launch_bounds__(TPB) void k_cg_b(KP P, double* __restrict__ x, double* __restrict__ r,
+                                             double* __restrict__ z, const double* __restrict__ p0,
+                                             const double* __restrict__ p1, const double* __restrict__ Ap,
+                                             const double* __restrict__ diag, int slot, double* part, DevScal* S) {
+    const int state = S->cg_state;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && slot >= 0) S->kind[slot] = state == CG_ITER ? KIND_ITER : KIND_NOOP;
+    if (state != CG_ITER) return;
+    const double* __restrict__ p = S->pidx ? p1 : p0;
     const double alpha = S->alpha;
     double v[2] = {0.0, 0.0};
     PEARL_CELL_LOOP(KK)
     x[i] += alpha * p[i];
     const double rr = r[i] - alpha * Ap[i];
+    const double zz = rr / diag[i];
     r[i] = rr;
-    const double z = rr / diag[i];
-    v[0] += rr * z;
+    z[i] = zz;
+    v[0] += rr * zz;
     v[1] += rr * rr;
     PEARL_CELL_LOOP_END
     if (block_reduce<2, 0u>(v, part, &S->counter, P.nb)) {
@@ -473,52 +571,9 @@ __global__ __launch_bounds__(TPB) void k_cg_update(KP P, double* __restrict__ x,
         S->beta = next / S->rz;
         S->rz = next;
         S->rr = v[1];
+        S->restart = 0;
+        cg_head(P.cg_max, S, S->cg_it + 1);
     }
-}
-
-// I3: p = z + beta p (:1079).
-template <int KK>
-__global__ __launch_bounds__(TPB) void k_cg_direction(KP P, const double* __restrict__ r,
-                                                     const double* __restrict__ diag, double* __restrict__ p,
-                                                     const DevScal* S) {
-    if (S->bad) return;
-    const double beta = S->beta;
-    PEARL_CELL_LOOP(KK)
-    p[i] = r[i] / diag[i] + beta * p[i];
-    PEARL_CELL_LOOP_END
-}
-
-// True residual at recursive convergence: r = rhs - A x; r.r (:1053-1055).
-template <int KK>
-__global__ __launch_bounds__(TPB) void k_true_residual(KP P, const double* __restrict__ chi,
-                                                      const double* __restrict__ K, const double* __restrict__ rhs,
-                                                      const double* __restrict__ x, double* __restrict__ r, double dt,
-                                                      double* part, DevScal* S) {
-    double v[1] = {0.0};
-    PEARL_CELL_LOOP(KK)
-    const Nbr nb = neighbours(P, i);
-    double w[4];
-    face_weights(P, K, i, nb, w);
-    const double rr = rhs[i] - apply_cell(P, x, chi[i], dt, i, nb, w);
-    r[i] = rr;
-    v[0] += rr * rr;
-    PEARL_CELL_LOOP_END
-    if (block_reduce<1, 0u>(v, part, &S->counter, P.nb)) S->rr = v[0];
-}
-
-// Restart from the true residual: z = r/diag; p = z; r.z (:1060-1064).
-template <int KK>
-__global__ __launch_bounds__(TPB) void k_cg_restart(KP P, const double* __restrict__ r,
-                                                   const double* __restrict__ diag, double* __restrict__ p,
-                                                   double* part, DevScal* S) {
-    double v[1] = {0.0};
-    PEARL_CELL_LOOP(KK)
-    const double rr = r[i];
-    const double z = rr / diag[i];
-    p[i] = z;
-    v[0] += rr * z;
-    PEARL_CELL_LOOP_END
-    if (block_reduce<1, 0u>(v, part, &S->counter, P.nb)) S->rz = v[0];
 }
 
 // F + A: flux-form carbon update, closure, mu reconstruction (:1082-1093),
@@ -531,6 +586,7 @@ __global__ __launch_bounds__(TPB) void k_finalize(KP P, const double* __restrict
                                                  const double* __restrict__ K, const double* __restrict__ x,
                                                  const double* __restrict__ c_old, double* __restrict__ c_new,
                                                  double* __restrict__ mu_new, double dt, double* part, DevScal* S) {
+    if (S->cg_state != CG_OK) return;
     double v[4] = {0.0, 1.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
     PEARL_CELL_LOOP(KK)
@@ -566,6 +622,7 @@ template <int KK>
 __global__ __launch_bounds__(TPB) void k_phase_only(KP P, const double* __restrict__ phi_t,
                                                    const double* __restrict__ mu_old, double* __restrict__ c_new,
                                                    double* __restrict__ mu_new, double* part, DevScal* S) {
+    if (!S->live) return;
     double v[4] = {0.0, 1.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
     PEARL_CELL_LOOP(KK)
@@ -610,11 +667,11 @@ __global__ __launch_bounds__(TPB) void k_state_reduce(KP P, const double* __rest
     v[2] += e;
     PEARL_CELL_LOOP_END
     v[0] = good ? 1.0 : 0.0;
-    if (!mix_ok) atomicOr(&S->err, ERRF_MIXTURE);
+    if (!mix_ok) atomicOr(&S->err, ERRF_MIXTURE_STATE);
     if (block_reduce<3, 2u>(v, part, &S->counter, P.nb)) {
-        S->valid = v[0];
-        S->mass = v[1];
-        S->energy = v[2];
+        S->e_valid = v[0];
+        S->e_mass = v[1];
+        S->e_energy = v[2];
     }
 }
 
@@ -725,5 +782,12 @@ __global__ __launch_bounds__(TPB) void k_shift(KP P, const double* __restrict__ 
 // Clears the error bits (a kernel rather than a memset so that the call is
 // safe inside captured graphs).
 __global__ void k_clear_flags(DevScal* S) { S->err = 0; }
+
+// Marks an attempt live without a phase proposal (implicit_diffusion on a
+// caller-supplied trial phi).
+__global__ void k_begin_solve(DevScal* S) {
+    S->live = 1;
+    S->cg_state = CG_IDLE;
+}
 
 }  // namespace pearl_gpu

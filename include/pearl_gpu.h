@@ -148,19 +148,21 @@ int pearl_get_layout(const pearl_ctx* ctx, int* threads_per_block, int* cells_pe
 /* Per-kernel timing with HIP events on the context's stream.  Kernel ids:
  * see PEARL_KERNEL_*.  When enabled, every launch is bracketed by events;
  * pearl_kernel_stats returns the launch count, total milliseconds and the
- * algorithmic HBM bytes of all launches of that kernel. */
-#define PEARL_KERNEL_PROPOSE 0
-#define PEARL_KERNEL_SETUP 1
-#define PEARL_KERNEL_RESIDUAL0 2
-#define PEARL_KERNEL_CG_AP 3
-#define PEARL_KERNEL_CG_UPDATE 4
-#define PEARL_KERNEL_CG_DIRECTION 5
-#define PEARL_KERNEL_TRUE_RESIDUAL 6
-#define PEARL_KERNEL_CG_RESTART 7
-#define PEARL_KERNEL_FINALIZE 8
-#define PEARL_KERNEL_STATE_REDUCE 9
-#define PEARL_KERNEL_WINDOW 10
-#define PEARL_KERNEL_COUNT 11
+ * algorithmic HBM bytes of all launches of that kernel.  Launches that a
+ * device-side decision turned into no-ops (a rejected phase increment, a
+ * PCG solve that had already finished) are counted under
+ * PEARL_KERNEL_CG_IDLE with zero bytes. */
+#define PEARL_KERNEL_PROPOSE 0        /* propose_phi */
+#define PEARL_KERNEL_SETUP 1          /* diffusion setup */
+#define PEARL_KERNEL_RESIDUAL0 2      /* initial residual, diagonal, tolerance */
+#define PEARL_KERNEL_CG_APPLY 3       /* direction update + A p + p.Ap */
+#define PEARL_KERNEL_CG_UPDATE 4      /* x, r, z updates + r.z, r.r */
+#define PEARL_KERNEL_TRUE_RESIDUAL 5  /* true residual at convergence (+ restart) */
+#define PEARL_KERNEL_CG_IDLE 6        /* speculative PCG launches that found the solve finished */
+#define PEARL_KERNEL_FINALIZE 7       /* flux update, closure, mu + acceptance reductions */
+#define PEARL_KERNEL_STATE_REDUCE 8   /* valid/mass/energy of the current state, diagnostics */
+#define PEARL_KERNEL_WINDOW 9         /* moving-window front, strip, shift */
+#define PEARL_KERNEL_COUNT 10
 int pearl_profile(pearl_ctx* ctx, int enable);
 int pearl_kernel_stats(pearl_ctx* ctx, int kernel, long long* launches, double* total_ms, double* bytes);
 const char* pearl_kernel_name(int kernel);

@@ -1,0 +1,137 @@
+"""The drop-in host program pearl_gpu against the reference program
+pearl_serial (built from source under oracle/_ref) on the same input file:
+same files, same step/rejection/PCG counts, fields within the FP64 bar, and
+checkpoints that each program restarts from."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, PKG_DIR, load_pkg
+
+import oracle as O
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "pearl_serial")),
+                                 reason="reference program not built")]
+EXE = os.path.join(PKG_DIR, "pearl_gpu")
+REF = os.path.join(ROOT, "oracle", "_ref", "pearl_serial")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_input(tmp, name, **ov):
+    load_pkg()
+    import importlib
+    inputs = importlib.import_module("pearl_amd.inputs")
+    p = os.path.join(tmp, name + ".in")
+    overrides = {"smooth_steps": 20, "smooth_min_steps": 20}
+    overrides.update(ov)
+    inputs.colony_input(p, 48, 40, 2, 10, 4.5e-10, output_dt=1.5e-10, overrides=overrides)
+    return p
+
+
+def run(exe, args, timeout=180):
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=timeout)
+    return r.returncode, r.stdout, r.stderr
+
+
+def read_status(path):
+    return dict(line.split("=", 1) for line in open(path).read().splitlines() if "=" in line)
+
+
+def load_checkpoint(inp, path):
+    R = O.Ref(inp)
+    R.read_checkpoint(path)  # the reference's own reader validates it
+    return R.get_state()
+
+
+def test_program_matches_reference(tmp_path):
+    tmp = str(tmp_path)
+    inp = make_input(tmp, "colony")
+    say("host program: reference run")
+    rc, out_r, err_r = run(REF, [inp, os.path.join(tmp, "ref")])
+    assert rc == 0, err_r
+    say("host program: gpu run")
+    rc, out_g, err_g = run(EXE, [inp, os.path.join(tmp, "gpu")])
+    assert rc == 0, err_g
+    ref, gpu = os.path.join(tmp, "ref"), os.path.join(tmp, "gpu")
+    assert sorted(os.listdir(ref)) == sorted(os.listdir(gpu))
+    for f in ["Input.in", "thermodynamics.txt", "initialization_report.txt", "COMPLETED.txt"]:
+        assert open(os.path.join(ref, f)).read() == open(os.path.join(gpu, f)).read(), f
+    for f in sorted(os.listdir(ref)):
+        if f.endswith(".xmf"):
+            assert open(os.path.join(ref, f)).read() == open(os.path.join(gpu, f)).read(), f
+    # diagnostics: same rows; sums agree to the summation-order bar
+    dr = np.genfromtxt(os.path.join(ref, "diagnostics.csv"), delimiter=",", names=True)
+    dg = np.genfromtxt(os.path.join(gpu, "diagnostics.csv"), delimiter=",", names=True)
+    assert dr.shape == dg.shape
+    for k in ["time", "step", "dt_next", "cg_last", "cg_peak", "rejected", "shift_distance", "c_min", "c_max",
+              "mu_min", "mu_max", "phi_sum_error"]:
+        assert np.array_equal(dr[k], dg[k]), k
+    assert np.allclose(dr["mean_carbon"], dg["mean_carbon"], rtol=1e-12, atol=0)
+    assert np.allclose(dr["reduced_energy"], dg["reduced_energy"], rtol=1e-12, atol=0)
+    # checkpoints: each loads through the reference reader (signature, closure, ledger)
+    phi_r, c_r, mu_r, sc_r = load_checkpoint(inp, os.path.join(ref, "checkpoint.h5"))
+    phi_g, c_g, mu_g, sc_g = load_checkpoint(inp, os.path.join(gpu, "checkpoint.h5"))
+    assert list(sc_r[[0, 1, 6, 8]]) == list(sc_g[[0, 1, 6, 8]])
+    assert np.abs(phi_r - phi_g).max() <= 1e-10
+    assert np.abs(mu_r - mu_g).max() <= 1e-5
+    # frame 0 (the initialized state) is bitwise identical
+    f0r = load_checkpoint(inp, os.path.join(ref, "frame_000000.h5"))
+    f0g = load_checkpoint(inp, os.path.join(gpu, "frame_000000.h5"))
+    for a, b in zip(f0r[:3], f0g[:3]):
+        assert np.array_equal(a, b)
+
+
+def test_restart_both_ways(tmp_path):
+    tmp = str(tmp_path)
+    inp = make_input(tmp, "colony")
+    inp_long = make_input(tmp, "colony_long")
+    with open(inp_long) as f:
+        text = f.read().replace("end_time = 4.5e-10", "end_time = 9e-10")
+    with open(inp_long, "w") as f:
+        f.write(text)
+    say("restart: reference run")
+    rc, _, err = run(REF, [inp, os.path.join(tmp, "ref")])
+    assert rc == 0, err
+    say("restart: gpu continues from the reference checkpoint")
+    rc, _, err = run(EXE, [inp_long, os.path.join(tmp, "gpu_cont"), "--restart", os.path.join(tmp, "ref", "checkpoint.h5")])
+    assert rc == 0, err
+    say("restart: reference continues from its own checkpoint")
+    rc, _, err = run(REF, [inp_long, os.path.join(tmp, "ref_cont"), "--restart", os.path.join(tmp, "ref", "checkpoint.h5")])
+    assert rc == 0, err
+    a = load_checkpoint(inp_long, os.path.join(tmp, "ref_cont", "checkpoint.h5"))
+    b = load_checkpoint(inp_long, os.path.join(tmp, "gpu_cont", "checkpoint.h5"))
+    assert list(a[3][[0, 6, 8]]) == list(b[3][[0, 6, 8]])
+    assert np.abs(a[2] - b[2]).max() <= 1e-5
+    say("restart: reference continues from the gpu checkpoint")
+    rc, _, err = run(EXE, [inp, os.path.join(tmp, "gpu")])
+    assert rc == 0, err
+    rc, _, err = run(REF, [inp_long, os.path.join(tmp, "ref_from_gpu"), "--restart", os.path.join(tmp, "gpu", "checkpoint.h5")])
+    assert rc == 0, err
+
+
+def test_cli_errors(tmp_path):
+    tmp = str(tmp_path)
+    inp = make_input(tmp, "colony")
+    os.makedirs(os.path.join(tmp, "exists"))
+    for args in [[inp, os.path.join(tmp, "exists")], [inp, os.path.join(tmp, "x1"), "--px", "2"],
+                 [inp, os.path.join(tmp, "x2"), "--bogus", "1"], [inp]]:
+        rc_r, _, err_r = run(REF, args)
+        rc_g, _, err_g = run(EXE, args)
+        assert rc_r == rc_g == 1
+        assert err_r.strip() == err_g.strip(), (args, err_r, err_g)
+
+
+def test_max_steps_stop(tmp_path):
+    tmp = str(tmp_path)
+    inp = make_input(tmp, "colony")
+    rc, _, err = run(EXE, [inp, os.path.join(tmp, "gpu"), "--max-steps", "3"])
+    assert rc == 0, err
+    st = read_status(os.path.join(tmp, "gpu", "STOPPED.txt"))
+    assert st["status"] == "USER_STOPPED" and st["step"] == "3"

@@ -1,0 +1,66 @@
+"""Device FP64 rounding as compiled for the solver: sqrt and division must
+be correctly rounded (bitwise equal to the host's IEEE results), and the
+constant-divisor quotient used for loop-invariant divisors must equal the
+IEEE quotient for the divisors and numerator ranges of the solver."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+LIB = os.path.join(ROOT, "tests", "hip", "librounding_probe.so")
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    L = C.CDLL(LIB)
+    L.probe_sqrt.argtypes = [_dp, _dp, C.c_int]
+    L.probe_div.argtypes = [_dp, _dp, _dp, C.c_int]
+    L.probe_div_const.argtypes = [_dp, C.c_double, C.c_double, _dp, C.c_int]
+    return L
+
+
+def p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def random_doubles(rng, n, lo=-300, hi=300):
+    m = rng.random(n) + 1.0
+    e = rng.integers(lo, hi, n)
+    return np.ldexp(m, e)
+
+
+def test_sqrt_correctly_rounded():
+    rng = np.random.default_rng(7)
+    x = np.concatenate([random_doubles(rng, 1 << 22), rng.random(1 << 20) * 1e-20, rng.random(1 << 20),
+                        np.array([0.0, 1.0, 2.0, 4.0, 1e-310, 5e-324, 1.7976931348623157e308])])
+    x = np.ascontiguousarray(x)
+    y = np.empty_like(x)
+    assert lib().probe_sqrt(p(x), p(y), x.size) == 0
+    ref = np.sqrt(x)
+    bad = np.flatnonzero(y != ref)
+    assert bad.size == 0, (bad.size, x[bad[:5]], y[bad[:5]], ref[bad[:5]])
+
+
+def test_division_correctly_rounded():
+    rng = np.random.default_rng(8)
+    a = np.ascontiguousarray(random_doubles(rng, 1 << 22) * rng.choice([-1, 1], 1 << 22))
+    b = np.ascontiguousarray(random_doubles(rng, 1 << 22))
+    q = np.empty_like(a)
+    assert lib().probe_div(p(a), p(b), p(q), a.size) == 0
+    assert np.array_equal(q, a / b)
+
+
+@pytest.mark.parametrize("d", [2.79e-9 * 2.79e-9, 1e-6, 3.0, 7.7841e-18, 1.2345678901234567])
+def test_constant_divisor_quotient(d):
+    rng = np.random.default_rng(9)
+    a = np.concatenate([random_doubles(rng, 1 << 22, -250, 250) * rng.choice([-1, 1], 1 << 22),
+                        rng.normal(size=1 << 20), rng.normal(size=1 << 20) * 1e-20])
+    a = np.ascontiguousarray(a)
+    q = np.empty_like(a)
+    assert lib().probe_div_const(p(a), d, 1.0 / d, p(q), a.size) == 0
+    bad = np.flatnonzero(q != a / d)
+    assert bad.size == 0, (bad.size, a[bad[:5]])
