@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one domain on one GPU; multi-GPU runs are not part of it",
@@ -158,6 +160,15 @@ def main():
             shifts[0] += int(sh)
             next_output[0] += cfg.output_dt
 
+    if args.copy_calibration and have_torch:
+        # A plain device-to-device copy of 1 GiB: FETCH_SIZE / WRITE_SIZE of
+        # this kernel scale the counters of the solver kernels.
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         one_step()
     log(f"warmup {args.warmup} steps done, t = {s.scalars().time:.4e}")

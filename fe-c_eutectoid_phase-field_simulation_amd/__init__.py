@@ -101,7 +101,9 @@ _lib = None
 EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state", "pearl_get_state",
            "pearl_advance", "pearl_run", "pearl_maybe_shift", "pearl_diagnostics", "pearl_propose_phi",
            "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
-           "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters"]
+           "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
+           "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
+           "pearl_batch_run"]
 
 
 def lib():
@@ -135,6 +137,13 @@ def lib():
         L.pearl_kernel_name.restype = C.c_char_p
         L.pearl_kernel_name.argtypes = [C.c_int]
         L.pearl_counters.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.pearl_batch_create.argtypes = [C.c_int, C.POINTER(Params), _dp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.pearl_batch_size.argtypes = [vp, C.POINTER(C.c_int)]
+        L.pearl_batch_set_state.argtypes = [vp, C.c_int, _dp, _dp, _dp, C.POINTER(Scalars)]
+        L.pearl_batch_get_state.argtypes = [vp, C.c_int, _dp, _dp, _dp, C.POINTER(Scalars)]
+        L.pearl_batch_run.argtypes = [vp, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                      C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(StepInfo),
+                                      C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -307,3 +316,59 @@ class Solver:
         a, r, cg = C.c_longlong(0), C.c_longlong(0), C.c_longlong(0)
         self._check(lib().pearl_counters(self._h, C.byref(a), C.byref(r), C.byref(cg)))
         return dict(attempts=a.value, rejected_attempts=r.value, cg_iterations=cg.value)
+
+
+class Batch(Solver):
+    """Many independent domains of any sizes in one context (one packed HBM
+    arena, one launch per kernel for all of them)."""
+
+    def __init__(self, params_list, mu_references, device: int = 0, cells_per_thread: int = 0,
+                 precision: int = FP64):
+        ps = [p if isinstance(p, Params) else params_from_dict(p) for p in params_list]
+        self.count = len(ps)
+        arr = (Params * self.count)(*ps)
+        mus = np.ascontiguousarray(mu_references, dtype=np.float64)
+        h = C.c_void_p()
+        rc = lib().pearl_batch_create(self.count, arr, _ptr(mus), precision, device, cells_per_thread, C.byref(h))
+        if rc != OK:
+            raise PearlError(rc, f"pearl_batch_create failed with status {rc}")
+        self._h = h
+        self.params = ps[0]
+        self.plist = ps
+        self.nx, self.ny = ps[0].nx, ps[0].ny
+
+    def set_domain_state(self, d, phi, carbon, mu, scalars=None):
+        p = self.plist[d]
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(3, p.ny, p.nx)
+        carbon = np.ascontiguousarray(carbon, dtype=np.float64).reshape(p.ny, p.nx)
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(p.ny, p.nx)
+        sc = None
+        if scalars is not None:
+            sc = scalars if isinstance(scalars, Scalars) else Scalars.from_list(scalars)
+        self._check(lib().pearl_batch_set_state(self._h, d, _ptr(phi), _ptr(carbon), _ptr(mu),
+                                                C.byref(sc) if sc is not None else None))
+
+    def get_domain_state(self, d):
+        p = self.plist[d]
+        phi = np.empty((3, p.ny, p.nx))
+        c = np.empty((p.ny, p.nx))
+        mu = np.empty((p.ny, p.nx))
+        sc = Scalars()
+        self._check(lib().pearl_batch_get_state(self._h, d, _ptr(phi), _ptr(c), _ptr(mu), C.byref(sc)))
+        return phi, c, mu, sc
+
+    def run_all(self, stop_times, max_steps):
+        n = self.count
+        st = np.ascontiguousarray(stop_times, dtype=np.float64)
+        ms = (C.c_longlong * n)(*[int(v) for v in max_steps])
+        taken = (C.c_longlong * n)()
+        cgt = (C.c_longlong * n)()
+        cgp = (C.c_int * n)()
+        last = (StepInfo * n)()
+        status = (C.c_int * n)()
+        rc = lib().pearl_batch_run(self._h, _ptr(st), ms, taken, cgt, cgp, last, status)
+        out = dict(rc=rc, taken=list(taken), cg_total=list(cgt), cg_peak=list(cgp), last=list(last),
+                   status=list(status))
+        if rc != OK:
+            out["error"] = lib().pearl_last_error(self._h).decode()
+        return out

@@ -1,9 +1,11 @@
-// C ABI of the FP64 per-step solver (include/pearl_gpu.h): context,
-// device-resident state, and the host-side step controller that restates
-// advance (pearl_serial.cpp:1119-1174), implicit_diffusion's PCG control
-// (:1044-1081) and maybe_shift (:2053-2071) on top of the kernels in
-// pearl_kernels.hpp.  The controller reads back only the few scalars each
-// decision needs; all field work stays on the device.
+// C ABI of the per-step solver (include/pearl_gpu.h).
+//
+// A context owns one or more domains packed in one HBM arena.  advance()
+// (pearl_serial.cpp:1119-1174) runs on the device: the host only enqueues
+// "rounds" of predicated kernels (pearl_kernels.hpp) and reads the device
+// scalars when a call has to return.  A round is captured once into a HIP
+// graph and replayed; with profiling on, rounds are launched eagerly and
+// bracketed by events so that each launch can be attributed to a kernel.
 #include "pearl_gpu.h"
 
 #include <hip/hip_runtime.h>
@@ -36,45 +38,46 @@ const char* kKernelNames[PEARL_KERNEL_COUNT] = {
 // R phi(3),c,mu.
 const double kWords[PEARL_KERNEL_COUNT] = {7, 10, 7, 6, 8, 7, 0, 10, 5, 0};
 
-// Launch accounting: a launch whose work depends on a device-side decision
-// is resolved after the next synchronisation.
-enum LaunchCond { COND_ALWAYS = 0, COND_LIVE = 1, COND_CG_OK = 2, COND_SLOT = 3 };
+// Device run counter of each kernel id (-1: none).
+const int kRunOf[PEARL_KERNEL_COUNT] = {RUN_PROPOSE,       RUN_SETUP, RUN_RESIDUAL0, RUN_CG_APPLY, RUN_CG_UPDATE,
+                                        RUN_TRUE_RESIDUAL, -1,        RUN_FINALIZE,  RUN_STATE,    -1};
 
 }  // namespace
 
 struct pearl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    pearl_params prm{};
-    double mu_ref = 0;
-    KP kp{};
-    int n = 0, k = 1;
-    // device buffers
+    int ndom = 0, k = 1;
+    std::vector<pearl_params> prm;
+    std::vector<KP> kp;             // host copy of the parameter blocks
+    std::vector<pearl_scalars> sc;  // host-side copy of the State scalars
+    long long total_cells = 0, total_blocks = 0, total_parts = 0;
+    // device memory
     double* mem = nullptr;
-    double *phi[2]{}, *c[2]{}, *mu[2]{};
-    int cur = 0;
-    double *chi = nullptr, *icpt = nullptr, *K = nullptr, *rhs = nullptr, *x = nullptr, *diag = nullptr;
-    double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *Ap = nullptr, *part = nullptr;
-    int cg_slots = 6;  // PCG slots enqueued per synchronisation
-    DevScal* S = nullptr;
-    DevScal* hS = nullptr;  // pinned host copy
-    pearl_scalars sc{};
-    bool e0_valid = false, m0_valid = false;
-    double e0 = 0, m0 = 0;
+    Bufs B{};
+    KP* d_kp = nullptr;
+    DevScal* d_S = nullptr;
+    int* d_blk2dom = nullptr;
+    int* d_kinds = nullptr;
+    DevScal* hS = nullptr;  // pinned mirror
     std::string err;
-    long long attempts = 0, rejected_attempts = 0, cg_iterations = 0;
+    int cg_slots = 6;  // PCG slots per round
+    // graph of one round
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_slots = 0;
     // profiling
     bool prof = false;
-    struct Pending {
-        int kid, cond, slot;
-        double cells;
-        hipEvent_t a, b;
-    };
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> free_events;
     long long launches[PEARL_KERNEL_COUNT]{};
     double ms[PEARL_KERNEL_COUNT]{};
     double bytes[PEARL_KERNEL_COUNT]{};
+    std::vector<unsigned> runs_seen;  // per domain x NKIND at the last harvest
+    struct Ev {
+        int kid, slot;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> pending;
+    std::vector<hipEvent_t> free_events;
 };
 
 namespace {
@@ -88,10 +91,20 @@ int hip_fail(pearl_ctx* c, hipError_t e, const char* where) {
     return fail(c, PEARL_ERR_DEVICE, std::string("HIP error in ") + where + ": " + hipGetErrorString(e));
 }
 
-#define HIP_CHECK(ctx, call)                                 \
-    do {                                                     \
-        hipError_t e_ = (call);                              \
+#define HIP_CHECK(ctx, call)                                   \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+#define PEARL_LAUNCH(ctx, kname, grid, ...)                                                              \
+    do {                                                                                                 \
+        switch ((ctx)->k) {                                                                              \
+            case 1: hipLaunchKernelGGL(kname<1>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
+            case 2: hipLaunchKernelGGL(kname<2>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
+            case 4: hipLaunchKernelGGL(kname<4>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
+            default: hipLaunchKernelGGL(kname<8>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
+        }                                                                                                \
     } while (0)
 
 hipEvent_t get_event(pearl_ctx* c) {
@@ -105,332 +118,194 @@ hipEvent_t get_event(pearl_ctx* c) {
     return e;
 }
 
-// Resolve the launches since the last synchronisation (the stream is
-// idle): a launch counts under its kernel, with its algorithmic bytes, if it
-// did work, and under PEARL_KERNEL_CG_IDLE otherwise.
-void harvest(pearl_ctx* c) {
-    const DevScal* h = c->hS;
-    for (auto& pd : c->pending) {
-        int kid = pd.kid;
-        bool real = true;
-        if (pd.cond == COND_LIVE) real = h->live != 0;
-        else if (pd.cond == COND_CG_OK) real = pd.slot == 1;
-        else if (pd.cond == COND_SLOT) {
-            const int kind = h->kind[pd.slot];
-            real = kind != KIND_NOOP;
-            if (kid == PEARL_KERNEL_CG_APPLY && kind == KIND_TRUE) kid = PEARL_KERNEL_TRUE_RESIDUAL;
-        }
-        if (!real) kid = PEARL_KERNEL_CG_IDLE;
-        c->launches[kid]++;
-        c->bytes[kid] += kWords[kid] * 8.0 * pd.cells;
-        if (pd.a) {
-            float t = 0;
-            (void)hipEventElapsedTime(&t, pd.a, pd.b);
-            c->ms[kid] += t;
-            c->free_events.push_back(pd.a);
-            c->free_events.push_back(pd.b);
-        }
-    }
-    c->pending.clear();
-}
-
-// Brackets one launch: records it for accounting and, when profiling, with
-// a pair of events on the context's stream.
-struct ProfScope {
-    pearl_ctx* c;
-    size_t idx;
-    ProfScope(pearl_ctx* cc, int kid, double cells, int cond = COND_ALWAYS, int slot = 0) : c(cc) {
-        pearl_ctx::Pending pd{kid, cond, slot, cells, nullptr, nullptr};
-        if (c->prof) {
-            pd.a = get_event(c);
-            pd.b = get_event(c);
-            (void)hipEventRecord(pd.a, c->stream);
-        }
-        c->pending.push_back(pd);
-        idx = c->pending.size() - 1;
-    }
-    ~ProfScope() {
-        if (c->prof) (void)hipEventRecord(c->pending[idx].b, c->stream);
-    }
-};
-
-dim3 grid_of(const pearl_ctx* c) { return dim3(static_cast<unsigned>(c->kp.nb)); }
-
-#define PEARL_DISPATCH(ctx, kname, grid, ...)                                                          \
-    do {                                                                                              \
-        switch ((ctx)->k) {                                                                           \
-            case 1: hipLaunchKernelGGL(kname<1>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
-            case 2: hipLaunchKernelGGL(kname<2>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
-            case 4: hipLaunchKernelGGL(kname<4>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
-            default: hipLaunchKernelGGL(kname<8>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
-        }                                                                                             \
-    } while (0)
-
-// Copy the device scalars to the pinned host block and wait.
-int sync_scalars(pearl_ctx* c) {
-    HIP_CHECK(c, hipMemcpyAsync(c->hS, c->S, sizeof(DevScal), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    HIP_CHECK(c, hipGetLastError());
-    harvest(c);
-    return PEARL_OK;
-}
-
-int err_from_flags(pearl_ctx* c, int flags) {
-    if (flags & ERRF_PROJECTION) return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
-    if (flags & ERRF_MIXTURE) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
-    if (flags & ERRF_NONFINITE_CARBON) return fail(c, PEARL_ERR_NONFINITE_CARBON, "Nonfinite total carbon");
-    return PEARL_OK;
-}
-
-double* cur_phi(pearl_ctx* c) { return c->phi[c->cur]; }
-double* cur_c(pearl_ctx* c) { return c->c[c->cur]; }
-double* cur_mu(pearl_ctx* c) { return c->mu[c->cur]; }
-double* trial_phi(pearl_ctx* c) { return c->phi[1 - c->cur]; }
-double* trial_c(pearl_ctx* c) { return c->c[1 - c->cur]; }
-double* trial_mu(pearl_ctx* c) { return c->mu[1 - c->cur]; }
-
-// valid_state / total_carbon / reduced_energy of the current state.
-int state_reduce(pearl_ctx* c, bool* valid, double* mass, double* energy, bool mixture_is_error) {
-    {
-        ProfScope ps(c, PEARL_KERNEL_STATE_REDUCE, c->n);
-        PEARL_DISPATCH(c, k_state_reduce, grid_of(c), c->kp, cur_phi(c), cur_c(c), cur_mu(c), c->part, c->S);
-    }
-    int st = sync_scalars(c);
-    if (st) return st;
-    if (mixture_is_error && (c->hS->err & ERRF_MIXTURE_STATE)) return err_from_flags(c, ERRF_MIXTURE);
-    if (c->hS->err) {
-        c->hS->err = 0;
-        hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(1), 0, c->stream, c->S);
-    }
-    if (valid) *valid = c->hS->e_valid > 0.5;
-    if (mass) *mass = c->hS->e_mass * c->kp.dx * c->kp.dy;
-    if (energy) *energy = c->hS->e_energy * c->kp.dx * c->kp.dy;
-    return PEARL_OK;
-}
-
-void launch_propose(pearl_ctx* c, double dt) {
-    ProfScope ps(c, PEARL_KERNEL_PROPOSE, c->n);
-    PEARL_DISPATCH(c, k_propose, grid_of(c), c->kp, cur_phi(c), cur_mu(c), trial_phi(c), dt, c->part, c->S);
-}
-
-int propose(pearl_ctx* c, double dt, double* change) {
-    launch_propose(c, dt);
-    int st = sync_scalars(c);
-    if (st) return st;
-    if (c->hS->err & ERRF_PROJECTION) return err_from_flags(c, ERRF_PROJECTION);
-    *change = c->hS->change;
-    return PEARL_OK;
-}
-
-// implicit_diffusion :1010-1094 on (cur c, cur mu, trial phi) -> trial c, mu,
-// fused with the acceptance reductions of the trial state.  The PCG loop is
-// controlled on the device (k_residual0 / k_cg_a / k_cg_b finishers); the
-// host enqueues `cg_slots` iterations at a time, each predicated on the
-// device state, and synchronises once per batch.
-struct DiffOut {
-    int ok = 0, it = 0;
-    double residual = 0, closure = 0;
-    bool finalized = false;
-};
-
-void launch_setup(pearl_ctx* c, double dt) {
-    const KP& P = c->kp;
-    const dim3 g = grid_of(c);
-    {
-        ProfScope ps(c, PEARL_KERNEL_SETUP, c->n, COND_LIVE);
-        PEARL_DISPATCH(c, k_setup, g, P, trial_phi(c), cur_c(c), cur_mu(c), c->chi, c->icpt, c->K, c->rhs, c->x, c->S);
-    }
-    {
-        ProfScope ps(c, PEARL_KERNEL_RESIDUAL0, c->n, COND_LIVE);
-        PEARL_DISPATCH(c, k_residual0, g, P, c->chi, c->K, c->rhs, c->x, c->diag, c->r, c->z, dt, c->part, c->S);
-    }
-}
-
-// Enqueue `slots` PCG iterations and the (predicated) finalize.
-void launch_cg_batch(pearl_ctx* c, double dt, int slots) {
-    const KP& P = c->kp;
-    const dim3 g = grid_of(c);
-    for (int j = 0; j < slots; ++j) {
-        {
-            ProfScope ps(c, PEARL_KERNEL_CG_APPLY, c->n, COND_SLOT, 2 * j);
-            PEARL_DISPATCH(c, k_cg_a, g, P, c->chi, c->K, c->rhs, c->x, c->diag, c->r, c->z, c->p0, c->p1, c->Ap, dt,
-                           2 * j, c->part, c->S);
-        }
-        {
-            ProfScope ps(c, PEARL_KERNEL_CG_UPDATE, c->n, COND_SLOT, 2 * j + 1);
-            PEARL_DISPATCH(c, k_cg_b, g, P, c->x, c->r, c->z, c->p0, c->p1, c->Ap, c->diag, 2 * j + 1, c->part, c->S);
-        }
-    }
-    ProfScope ps(c, PEARL_KERNEL_FINALIZE, c->n, COND_CG_OK, 0);
-    PEARL_DISPATCH(c, k_finalize, g, P, trial_phi(c), c->chi, c->icpt, c->K, c->x, cur_c(c), trial_c(c), trial_mu(c),
-                   dt, c->part, c->S);
-}
-
-// After the first synchronisation of an attempt: finish the PCG solve if
-// the enqueued iterations were not enough, and read its outcome.
-int sync_attempt(pearl_ctx* c);
-
-int finish_diffusion(pearl_ctx* c, double dt, DiffOut& out) {
-    while (c->hS->cg_state == CG_ITER || c->hS->cg_state == CG_TRUE) {
-        launch_cg_batch(c, dt, c->cg_slots);
-        const int st = sync_attempt(c);
-        if (st) return st;
-    }
-    const DevScal* h = c->hS;
-    out.it = h->cg_it;
-    out.residual = h->cg_res;
-    c->cg_iterations += h->cg_it;
-    c->cg_slots = std::min(28, std::max(2, h->cg_it + 2));
-    if (h->cg_state != CG_OK) return PEARL_OK;
-    out.closure = h->closure;
-    out.ok = out.closure <= c->prm.closure_tol;
-    out.finalized = true;
-    return PEARL_OK;
-}
-
-// Marks the last finalize launch of the pending list as the one that ran.
-void mark_final_finalize(pearl_ctx* c) {
-    for (auto it = c->pending.rbegin(); it != c->pending.rend(); ++it)
-        if (it->cond == COND_CG_OK) {
-            it->slot = 1;
-            break;
-        }
-}
-
-// Synchronise after a batch.  The finalize launch runs only in the batch in
-// which the solve finished, and it is the last one enqueued in that batch.
-int sync_attempt(pearl_ctx* c) {
-    HIP_CHECK(c, hipMemcpyAsync(c->hS, c->S, sizeof(DevScal), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    HIP_CHECK(c, hipGetLastError());
-    if (c->hS->cg_state == CG_OK) mark_final_finalize(c);
-    harvest(c);
-    return PEARL_OK;
-}
-
-int diffusion(pearl_ctx* c, double dt, DiffOut& out) {
-    hipLaunchKernelGGL(k_begin_solve, dim3(1), dim3(1), 0, c->stream, c->S);
-    launch_setup(c, dt);
-    launch_cg_batch(c, dt, c->cg_slots);
-    int st = sync_attempt(c);
-    if (st) return st;
-    if (c->hS->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
-    return finish_diffusion(c, dt, out);
-}
-
-int advance_impl(pearl_ctx* c, double stop_time, pearl_step_info* info) {
-    const pearl_params& prm = c->prm;
-    pearl_scalars& s = c->sc;
-    double dt = s.dt;
-    if (prm.dt_max < dt) dt = prm.dt_max;
-    if (stop_time - s.time < dt) dt = stop_time - s.time;
-    std::memset(info, 0, sizeof(*info));
-    info->why = -1;
-    if (!(dt > 0) || !std::isfinite(dt)) return fail(c, PEARL_ERR_INVALID_TIME, "invalid advance time");
-    bool need_state = (prm.check_energy && !c->e0_valid) || (!prm.phase_only && !c->m0_valid);
-    int why = PEARL_WHY_PHASE;
-    for (int attempt = 0; attempt <= prm.max_retries; ++attempt) {
-        c->attempts++;
-        pearl_step_info out{};
-        out.retries = attempt;
-        out.accepted_dt = dt;
-        out.why = -1;
-        // One batch: [state reductions] propose, setup, residual0, PCG
-        // iterations, finalize; everything after propose is predicated on
-        // the device-side phase-increment decision.
-        if (need_state) {
-            ProfScope ps(c, PEARL_KERNEL_STATE_REDUCE, c->n);
-            PEARL_DISPATCH(c, k_state_reduce, grid_of(c), c->kp, cur_phi(c), cur_c(c), cur_mu(c), c->part, c->S);
-        }
-        launch_propose(c, dt);
-        if (prm.phase_only) {
-            ProfScope ps(c, PEARL_KERNEL_FINALIZE, c->n, COND_LIVE);
-            PEARL_DISPATCH(c, k_phase_only, grid_of(c), c->kp, trial_phi(c), cur_mu(c), trial_c(c), trial_mu(c),
-                           c->part, c->S);
+// Enqueue one round: begin, state, propose, setup, residual0, slots x
+// (cg_a, cg_b), finalize, phase-only, decide.  With `timed`, each cell
+// kernel is bracketed by events and writes its work record into
+// kinds[slot].
+void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide) {
+    const dim3 g(static_cast<unsigned>(c->total_blocks));
+    const dim3 gd((c->ndom + 63) / 64);
+    int slot = 0;
+    auto ev = [&](int kid, auto&& fn) {
+        if (timed) {
+            hipEvent_t a = get_event(c), b = get_event(c);
+            (void)hipEventRecord(a, c->stream);
+            fn(slot);
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({kid, slot, a, b});
         } else {
-            launch_setup(c, dt);
-            launch_cg_batch(c, dt, c->cg_slots);
+            fn(-1);
         }
-        int st = sync_attempt(c);
-        if (st) return st;
-        const DevScal* h = c->hS;
-        if (need_state) {
-            if (prm.check_energy && (h->err & ERRF_MIXTURE_STATE)) return err_from_flags(c, ERRF_MIXTURE);
-            if (prm.check_energy) {
-                c->e0 = h->e_energy * c->kp.dx * c->kp.dy;
-                c->e0_valid = true;
-            }
-            c->m0 = h->e_mass * c->kp.dx * c->kp.dy;
-            c->m0_valid = true;
-            need_state = false;
+        ++slot;
+    };
+    if (with_begin) hipLaunchKernelGGL(k_begin, gd, dim3(64), 0, c->stream, c->B);
+    ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, c->B, s); });
+    ev(PEARL_KERNEL_PROPOSE, [&](int s) { PEARL_LAUNCH(c, k_propose, g, c->B, s); });
+    ev(PEARL_KERNEL_SETUP, [&](int s) { PEARL_LAUNCH(c, k_setup, g, c->B, s); });
+    ev(PEARL_KERNEL_RESIDUAL0, [&](int s) { PEARL_LAUNCH(c, k_residual0, g, c->B, s); });
+    for (int j = 0; j < slots; ++j) {
+        ev(PEARL_KERNEL_CG_APPLY, [&](int s) { PEARL_LAUNCH(c, k_cg_a, g, c->B, s); });
+        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { PEARL_LAUNCH(c, k_cg_b, g, c->B, s); });
+    }
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, c->B, s); });
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, c->B, s); });
+    if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, c->stream, c->B);
+}
+
+// Attribute the timed launches since the last harvest (stream idle) and
+// the per-domain work counters to the kernels.
+void harvest(pearl_ctx* c) {
+    if (!c->pending.empty()) {
+        std::vector<int> kinds(MAX_SLOTS, 0);
+        (void)hipMemcpy(kinds.data(), c->d_kinds, MAX_SLOTS * sizeof(int), hipMemcpyDeviceToHost);
+        for (auto& e : c->pending) {
+            int kid = e.kid;
+            const int kind = e.slot < MAX_SLOTS ? kinds[e.slot] : 0;
+            if (kind == 0) kid = PEARL_KERNEL_CG_IDLE;
+            else if (kid == PEARL_KERNEL_CG_APPLY && kind == 2) kid = PEARL_KERNEL_TRUE_RESIDUAL;
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e.a, e.b);
+            c->ms[kid] += t;
+            c->launches[kid]++;
+            c->free_events.push_back(e.a);
+            c->free_events.push_back(e.b);
         }
-        if (h->err & ERRF_PROJECTION) return err_from_flags(c, ERRF_PROJECTION);
-        const double e0 = prm.check_energy ? c->e0 : 0;
-        const double mass0 = prm.phase_only ? 0 : c->m0;
-        const double change = h->change;
-        out.max_phase_change = change;
-        bool good = std::isfinite(change) && change <= prm.max_phase_change;
-        why = PEARL_WHY_PHASE;
-        double mass = 0, energy = 0, valid = 0;
-        if (good) {
-            if (h->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
-            if (!prm.phase_only) {
-                DiffOut d;
-                st = finish_diffusion(c, dt, d);
-                if (st) return st;
-                good = d.ok != 0;
-                out.cg_iterations = d.it;
-                out.linear_residual = d.residual;
-                out.closure = d.closure;
-                why = PEARL_WHY_LINEAR;
-            }
-            // (sum*dx)*dy as total_carbon :787 and reduced_energy :820
-            valid = c->hS->valid;
-            mass = c->hS->mass * c->kp.dx * c->kp.dy;
-            energy = c->hS->energy * c->kp.dx * c->kp.dy;
-        }
-        if (good) {
-            good = valid > 0.5;
-            why = PEARL_WHY_BOUNDS;
-        }
-        if (good && !prm.phase_only) {
-            const double lim = prm.nx * prm.ny * prm.dx * prm.dy * 1e-6;
-            good = std::fabs(mass - mass0) <= 1e-11 * std::max(std::fabs(mass0), lim);
-            why = PEARL_WHY_MASS;
-        }
-        if (good && prm.check_energy) {
-            const double e1 = energy;
-            out.energy_change = e1 - e0;
-            const double tol = prm.energy_atol + prm.energy_rtol * std::max(std::fabs(e0), prm.sigma * std::max(prm.nx * prm.dx, prm.ny * prm.dy));
-            good = std::isfinite(e1) && e1 <= e0 + tol;
-            why = PEARL_WHY_ENERGY;
-        }
-        if (good) {
-            c->cur = 1 - c->cur;
-            s.time = s.time + dt;
-            s.step = s.step + 1;
-            s.rejected = s.rejected + attempt;
-            s.dt = std::min(prm.dt_max, dt * 1.1);
-            if (prm.check_energy) {
-                c->e0 = energy;
-                c->e0_valid = true;
-            }
-            if (!prm.phase_only) {
-                c->m0 = mass;
-                c->m0_valid = true;
-            }
-            *info = out;
-            return PEARL_OK;
-        }
-        c->rejected_attempts++;
-        dt *= .5;
-        if (dt < prm.dt_min) {
-            info->why = why;
-            return fail(c, PEARL_ERR_DT_MIN, std::string("step rejected below dt_min: ") + kWhy[why]);
+        c->pending.clear();
+    }
+    // bytes from the device work counters (exact per domain)
+    for (int d = 0; d < c->ndom; ++d) {
+        for (int kid = 0; kid < PEARL_KERNEL_COUNT; ++kid) {
+            const int r = kRunOf[kid];
+            if (r < 0) continue;
+            const unsigned now = c->hS[d].runs[r];
+            unsigned& seen = c->runs_seen[static_cast<size_t>(d) * NKIND + r];
+            c->bytes[kid] += kWords[kid] * 8.0 * c->kp[d].n * static_cast<double>(now - seen);
+            if (!c->prof) c->launches[kid] += now - seen;
+            seen = now;
         }
     }
-    info->why = why;
-    return fail(c, PEARL_ERR_RETRY_LIMIT, std::string("step retry limit: ") + kWhy[why]);
+}
+
+int read_scalars(pearl_ctx* c) {
+    HIP_CHECK(c, hipMemcpyAsync(c->hS, c->d_S, c->ndom * sizeof(DevScal), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, hipGetLastError());
+    harvest(c);
+    return PEARL_OK;
+}
+
+int write_scalars(pearl_ctx* c) {
+    HIP_CHECK(c, hipMemcpyAsync(c->d_S, c->hS, c->ndom * sizeof(DevScal), hipMemcpyHostToDevice, c->stream));
+    return PEARL_OK;
+}
+
+// Replay `rounds` rounds: from the captured graph, or eagerly with events
+// (profiling; the work records are read after every round).
+int run_rounds(pearl_ctx* c, int rounds) {
+    if (c->prof) {
+        for (int r = 0; r < rounds; ++r) {
+            hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
+            enqueue_round(c, c->cg_slots, true, true, true);
+            if (r + 1 < rounds) {
+                const int st = read_scalars(c);
+                if (st) return st;
+            }
+        }
+        return PEARL_OK;
+    }
+    if (!c->graph_exec || c->graph_slots != c->cg_slots) {
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_round(c, c->cg_slots, false, true, true);
+        HIP_CHECK(c, hipStreamEndCapture(c->stream, &c->graph));
+        HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_slots = c->cg_slots;
+    }
+    for (int r = 0; r < rounds; ++r) HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    return PEARL_OK;
+}
+
+int status_of(pearl_ctx* c, int d) {
+    const DevScal& h = c->hS[d];
+    const int why = (h.why >= 0 && h.why < 5) ? h.why : 0;
+    switch (h.status) {
+        case ST_OK: return PEARL_OK;
+        case ST_INVALID_TIME: return fail(c, PEARL_ERR_INVALID_TIME, "invalid advance time");
+        case ST_DT_MIN: return fail(c, PEARL_ERR_DT_MIN, std::string("step rejected below dt_min: ") + kWhy[why]);
+        case ST_RETRY_LIMIT: return fail(c, PEARL_ERR_RETRY_LIMIT, std::string("step retry limit: ") + kWhy[why]);
+        case ST_MIXTURE: return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+        case ST_PROJECTION: return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+        default: return fail(c, PEARL_ERR_DEVICE, "unknown device status");
+    }
+}
+
+void info_of(const DevScal& h, pearl_step_info* info) {
+    info->accepted_dt = h.i_dt;
+    info->max_phase_change = h.i_change;
+    info->linear_residual = h.i_residual;
+    info->closure = h.i_closure;
+    info->energy_change = h.i_energy_change;
+    info->cg_iterations = h.i_cg;
+    info->retries = h.i_retries;
+    info->why = h.status == ST_OK ? -1 : h.why;
+}
+
+// Run every domain until it has taken max_steps[d] accepted steps or reached
+// stop_time[d] (the step loop between two output times).  Rounds are
+// replayed in batches between synchronisations; domains that finish idle.
+int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_steps) {
+    int st = read_scalars(c);
+    if (st) return st;
+    long long most = 0;
+    for (int d = 0; d < c->ndom; ++d) {
+        DevScal& h = c->hS[d];
+        h.status = ST_OK;
+        h.why = -1;
+        h.taken = 0;
+        h.target_steps = max_steps[d];
+        h.stop_time = stop_time[d];
+        h.time = c->sc[d].time;
+        h.dt_state = c->sc[d].dt;
+        h.step = c->sc[d].step;
+        h.rejected = c->sc[d].rejected;
+        h.cg_total = 0;
+        h.cg_peak = 0;
+        h.i_dt = h.i_change = h.i_residual = h.i_closure = h.i_energy_change = 0;
+        h.i_cg = h.i_retries = 0;
+        const bool work = max_steps[d] > 0 && h.time < stop_time[d];
+        h.phase = work ? PH_START : PH_IDLE;
+        if (work) most = std::max(most, max_steps[d]);
+    }
+    st = write_scalars(c);
+    if (st) return st;
+    // Rounds per synchronisation: about one accepted step per round; the
+    // batch grows while domains are still running.
+    long long batch = std::min<long long>(std::max<long long>(most, 1), 4);
+    while (true) {
+        st = run_rounds(c, static_cast<int>(batch));
+        if (st) return st;
+        st = read_scalars(c);
+        if (st) return st;
+        bool busy = false;
+        for (int d = 0; d < c->ndom; ++d) busy = busy || c->hS[d].phase != PH_IDLE;
+        if (!busy) break;
+        batch = std::min<long long>(batch * 2, 64);
+    }
+    int peak = 0;
+    for (int d = 0; d < c->ndom; ++d) {
+        const DevScal& h = c->hS[d];
+        c->sc[d].time = h.time;
+        c->sc[d].dt = h.dt_state;
+        c->sc[d].step = h.step;
+        c->sc[d].rejected = h.rejected;
+        peak = std::max(peak, h.cg_peak);
+    }
+    // PCG slots per round: the largest solve plus the true-residual check.
+    if (peak > 0) c->cg_slots = std::min(24, std::max(3, peak + 2));
+    return PEARL_OK;
 }
 
 bool params_ok(const pearl_params* p) {
@@ -443,39 +318,14 @@ bool params_ok(const pearl_params* p) {
     return true;
 }
 
-}  // namespace
-
-extern "C" {
-
-int pearl_create(const pearl_params* params, double mu_reference, int precision, int device, int cells_per_thread,
-                 pearl_ctx** out) {
-    if (!out) return PEARL_ERR_ARGUMENT;
-    *out = nullptr;
-    if (!params_ok(params)) return PEARL_ERR_ARGUMENT;
-    if (precision != PEARL_FP64) return PEARL_ERR_ARGUMENT;
-    auto* c = new pearl_ctx();
-    c->device = device;
-    c->prm = *params;
-    c->mu_ref = mu_reference;
-    c->n = params->nx * params->ny;
-    int k = cells_per_thread;
-    if (k <= 0) k = c->n >= (1 << 21) ? 4 : 1;
-    if (k != 1 && k != 2 && k != 4 && k != 8) {
-        delete c;
-        return PEARL_ERR_ARGUMENT;
-    }
-    c->k = k;
-    if (hipSetDevice(device) != hipSuccess) {
-        delete c;
-        return PEARL_ERR_DEVICE;
-    }
-    // Kernel parameter block: every derived constant uses the reference's
-    // own expression.
-    KP& P = c->kp;
+// Parameter block of one domain: every derived constant uses the
+// reference's own expression.
+KP make_kp(const pearl_params* params, double mu_reference, int k) {
+    KP P{};
     P.nx = params->nx;
     P.ny = params->ny;
-    P.n = c->n;
-    P.nb = (c->n + TPB * k - 1) / (TPB * k);
+    P.n = params->nx * params->ny;
+    P.nb = (P.n + TPB * k - 1) / (TPB * k);
     P.dx = params->dx;
     P.dy = params->dy;
     P.dx2 = params->dx * params->dx;
@@ -499,6 +349,7 @@ int pearl_create(const pearl_params* params, double mu_reference, int precision,
     P.face_coef = 2 * eps * sigma;
     P.eps_sigma = eps * sigma;
     P.eps = eps;
+    P.sigma = sigma;
     P.Vm = params->Vm;
     auto psi = [&](int p, double mu) {
         const double d = mu - params->B[p];
@@ -508,75 +359,162 @@ int pearl_create(const pearl_params* params, double mu_reference, int precision,
     P.psiref[1] = psi(1, mu_reference) - psi(2, mu_reference);
     P.psiref[2] = 0;
     P.mu_ref = mu_reference;
+    P.dt_max = params->dt_max;
+    P.dt_min = params->dt_min;
     P.max_phase_change = params->max_phase_change;
     P.cg_rtol = params->cg_rtol;
     P.cg_atol = params->cg_atol;
+    P.closure_tol = params->closure_tol;
+    P.energy_rtol = params->energy_rtol;
+    P.energy_atol = params->energy_atol;
+    P.mass_floor = params->nx * params->ny * params->dx * params->dy * 1e-6;
+    P.energy_scale = params->sigma * std::max(params->nx * params->dx, params->ny * params->dy);
     P.cg_max = params->cg_max;
+    P.max_retries = params->max_retries;
+    P.check_energy = params->check_energy;
     P.phase_only = params->phase_only;
+    return P;
+}
 
-    const size_t n = static_cast<size_t>(c->n);
-    const size_t plane = (n + 31) & ~static_cast<size_t>(31);  // 256-byte aligned planes
-    const size_t nparts = 8 * static_cast<size_t>(P.nb) + 64;
-    const size_t total = plane * 21 + nparts;
-    if (hipMalloc(&c->mem, total * sizeof(double)) != hipSuccess) {
-        delete c;
-        return PEARL_ERR_ALLOC;
+int ctx_check(pearl_ctx* c, int d) {
+    if (!c || d < 0 || d >= c->ndom) return PEARL_ERR_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    return PEARL_OK;
+}
+
+double* dom_phi(pearl_ctx* c, int d, int buf) { return c->B.phi[buf] + 3 * c->kp[d].off; }
+double* dom_c(pearl_ctx* c, int d, int buf) { return c->B.c[buf] + c->kp[d].off; }
+double* dom_mu(pearl_ctx* c, int d, int buf) { return c->B.mu[buf] + c->kp[d].off; }
+
+// valid/mass/energy of domain d's current state.
+int state_direct(pearl_ctx* c, int d, bool mixture_is_error, double* valid, double* mass, double* energy) {
+    const KP& P = c->kp[d];
+    PEARL_LAUNCH(c, k_state_direct, dim3(P.nb), c->B, d);
+    const int st = read_scalars(c);
+    if (st) return st;
+    const DevScal& h = c->hS[d];
+    if (mixture_is_error && (h.err & ERRF_MIXTURE_STATE)) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+    if (valid) *valid = h.e_valid;
+    if (mass) *mass = h.e_mass * P.dx * P.dy;
+    if (energy) *energy = h.e_energy * P.dx * P.dy;
+    return PEARL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pearl_batch_create(int count, const pearl_params* params, const double* mu_references, int precision, int device,
+                       int cells_per_thread, pearl_ctx** out) {
+    if (!out || count < 1 || !params || !mu_references) return PEARL_ERR_ARGUMENT;
+    *out = nullptr;
+    if (precision != PEARL_FP64) return PEARL_ERR_ARGUMENT;
+    long long cells = 0;
+    for (int d = 0; d < count; ++d) {
+        if (!params_ok(&params[d])) return PEARL_ERR_ARGUMENT;
+        cells += static_cast<long long>(params[d].nx) * params[d].ny;
     }
+    int k = cells_per_thread;
+    if (k <= 0) k = cells >= (1LL << 21) ? 4 : 1;
+    if (k != 1 && k != 2 && k != 4 && k != 8) return PEARL_ERR_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
+    auto* c = new pearl_ctx();
+    c->device = device;
+    c->ndom = count;
+    c->k = k;
+    c->prm.assign(params, params + count);
+    c->sc.assign(count, pearl_scalars{});
+    c->runs_seen.assign(static_cast<size_t>(count) * NKIND, 0u);
+    // Pack the domains; each slice starts on a 64-cell (512-byte) boundary.
+    long long off = 0, boff = 0, poff = 0;
+    std::vector<int> blk2dom;
+    for (int d = 0; d < count; ++d) {
+        KP P = make_kp(&params[d], mu_references[d], k);
+        P.off = off;
+        P.boff = static_cast<int>(boff);
+        P.poff = poff;
+        off += (static_cast<long long>(P.n) + 63) / 64 * 64;
+        boff += P.nb;
+        poff += 8LL * P.nb + 8;
+        blk2dom.insert(blk2dom.end(), P.nb, d);
+        c->kp.push_back(P);
+    }
+    c->total_cells = off;
+    c->total_blocks = boff;
+    c->total_parts = poff;
+    const size_t plane = static_cast<size_t>(off);
+    const size_t total = plane * (3 * 2 + 2 + 2 + 11) + static_cast<size_t>(poff) + 64;
+    auto bail = [&](int code) {
+        pearl_destroy(c);
+        return code;
+    };
+    if (hipMalloc(&c->mem, total * sizeof(double)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
     double* q = c->mem;
     for (int b = 0; b < 2; ++b) {
-        c->phi[b] = q;
+        c->B.phi[b] = q;
         q += 3 * plane;
     }
     for (int b = 0; b < 2; ++b) {
-        c->c[b] = q;
+        c->B.c[b] = q;
         q += plane;
     }
     for (int b = 0; b < 2; ++b) {
-        c->mu[b] = q;
+        c->B.mu[b] = q;
         q += plane;
     }
-    c->chi = q; q += plane;
-    c->icpt = q; q += plane;
-    c->K = q; q += plane;
-    c->rhs = q; q += plane;
-    c->x = q; q += plane;
-    c->diag = q; q += plane;
-    c->r = q; q += plane;
-    c->z = q; q += plane;
-    c->p0 = q; q += plane;
-    c->p1 = q; q += plane;
-    c->Ap = q; q += plane;
-    c->part = q;
-    if (hipMalloc(&c->S, sizeof(DevScal)) != hipSuccess || hipHostMalloc(&c->hS, sizeof(DevScal)) != hipSuccess) {
-        pearl_destroy(c);
-        return PEARL_ERR_ALLOC;
+    double** work[] = {&c->B.chi, &c->B.icpt, &c->B.K, &c->B.rhs, &c->B.x, &c->B.diag,
+                       &c->B.r,   &c->B.z,    &c->B.p0, &c->B.p1, &c->B.Ap};
+    for (double** w : work) {
+        *w = q;
+        q += plane;
     }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        pearl_destroy(c);
-        return PEARL_ERR_DEVICE;
-    }
-    (void)hipMemsetAsync(c->S, 0, sizeof(DevScal), c->stream);
-    (void)hipMemsetAsync(c->mem, 0, total * sizeof(double), c->stream);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) {
-        pearl_destroy(c);
-        return PEARL_ERR_DEVICE;
-    }
-    std::memset(c->hS, 0, sizeof(DevScal));
+    c->B.part = q;
+    if (hipMalloc(&c->d_kp, count * sizeof(KP)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+    if (hipMalloc(&c->d_S, count * sizeof(DevScal)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+    if (hipMalloc(&c->d_blk2dom, blk2dom.size() * sizeof(int)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+    if (hipMalloc(&c->d_kinds, MAX_SLOTS * sizeof(int)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+    if (hipHostMalloc(&c->hS, count * sizeof(DevScal)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(PEARL_ERR_DEVICE);
+    c->B.kp = c->d_kp;
+    c->B.S = c->d_S;
+    c->B.blk2dom = c->d_blk2dom;
+    c->B.kinds = c->d_kinds;
+    c->B.ndom = count;
+    std::memset(c->hS, 0, count * sizeof(DevScal));
+    for (int d = 0; d < count; ++d) c->hS[d].phase = PH_IDLE;
+    if (hipMemcpyAsync(c->d_kp, c->kp.data(), count * sizeof(KP), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->d_blk2dom, blk2dom.data(), blk2dom.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->d_S, c->hS, count * sizeof(DevScal), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->mem, 0, total * sizeof(double), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->d_kinds, 0, MAX_SLOTS * sizeof(int), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(PEARL_ERR_DEVICE);
     *out = c;
     return PEARL_OK;
+}
+
+int pearl_create(const pearl_params* params, double mu_reference, int precision, int device, int cells_per_thread,
+                 pearl_ctx** out) {
+    return pearl_batch_create(1, params, &mu_reference, precision, device, cells_per_thread, out);
 }
 
 int pearl_destroy(pearl_ctx* c) {
     if (!c) return PEARL_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& pd : c->pending) {
-        (void)hipEventDestroy(pd.a);
-        (void)hipEventDestroy(pd.b);
+    for (auto& e : c->pending) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->mem) (void)hipFree(c->mem);
-    if (c->S) (void)hipFree(c->S);
+    if (c->d_kp) (void)hipFree(c->d_kp);
+    if (c->d_S) (void)hipFree(c->d_S);
+    if (c->d_blk2dom) (void)hipFree(c->d_blk2dom);
+    if (c->d_kinds) (void)hipFree(c->d_kinds);
     if (c->hS) (void)hipHostFree(c->hS);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -585,73 +523,124 @@ int pearl_destroy(pearl_ctx* c) {
 
 const char* pearl_last_error(const pearl_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-int pearl_set_state(pearl_ctx* c, const double* phi, const double* carbon, const double* mu,
-                    const pearl_scalars* scalars) {
-    if (!c) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    const size_t n = static_cast<size_t>(c->n);
-    if (phi) HIP_CHECK(c, hipMemcpyAsync(cur_phi(c), phi, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (carbon) HIP_CHECK(c, hipMemcpyAsync(cur_c(c), carbon, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (mu) HIP_CHECK(c, hipMemcpyAsync(cur_mu(c), mu, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    if (scalars) c->sc = *scalars;
-    c->e0_valid = false;
-    c->m0_valid = false;
+int pearl_batch_size(const pearl_ctx* c, int* count) {
+    if (!c || !count) return PEARL_ERR_ARGUMENT;
+    *count = c->ndom;
     return PEARL_OK;
 }
 
-int pearl_get_state(pearl_ctx* c, double* phi, double* carbon, double* mu, pearl_scalars* scalars) {
-    if (!c) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    const size_t n = static_cast<size_t>(c->n);
-    if (phi) HIP_CHECK(c, hipMemcpyAsync(phi, cur_phi(c), 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (carbon) HIP_CHECK(c, hipMemcpyAsync(carbon, cur_c(c), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (mu) HIP_CHECK(c, hipMemcpyAsync(mu, cur_mu(c), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    if (scalars) *scalars = c->sc;
-    return PEARL_OK;
-}
-
-int pearl_advance(pearl_ctx* c, double stop_time, pearl_step_info* info) {
-    if (!c || !info) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    return advance_impl(c, stop_time, info);
-}
-
-int pearl_run(pearl_ctx* c, double stop_time, long long max_steps, long long* taken, long long* cg_total,
-              int* cg_peak, pearl_step_info* last) {
-    if (!c) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    long long t = 0;
-    pearl_step_info info{};
-    int st = PEARL_OK;
-    while (c->sc.time < stop_time && t < max_steps) {
-        st = advance_impl(c, stop_time, &info);
-        if (st) break;
-        ++t;
-        if (cg_total) *cg_total += info.cg_iterations;
-        if (cg_peak) *cg_peak = std::max(*cg_peak, info.cg_iterations);
+int pearl_batch_set_state(pearl_ctx* c, int d, const double* phi, const double* carbon, const double* mu,
+                          const pearl_scalars* scalars) {
+    int st = ctx_check(c, d);
+    if (st) return st;
+    st = read_scalars(c);
+    if (st) return st;
+    DevScal& h = c->hS[d];
+    const size_t n = static_cast<size_t>(c->kp[d].n);
+    if (phi) HIP_CHECK(c, hipMemcpyAsync(dom_phi(c, d, h.cur), phi, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (carbon) HIP_CHECK(c, hipMemcpyAsync(dom_c(c, d, h.cur), carbon, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (mu) HIP_CHECK(c, hipMemcpyAsync(dom_mu(c, d, h.cur), mu, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (scalars) c->sc[d] = *scalars;
+    h.time = c->sc[d].time;
+    h.dt_state = c->sc[d].dt;
+    h.step = c->sc[d].step;
+    h.rejected = c->sc[d].rejected;
+    // the cached e0 / mass0 belong to the previous fields
+    if (phi || carbon || mu) {
+        h.e0_valid = 0;
+        h.m0_valid = 0;
     }
+    h.err = 0;
+    h.status = ST_OK;
+    h.phase = PH_IDLE;
+    st = write_scalars(c);
+    if (st) return st;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return PEARL_OK;
+}
+
+int pearl_batch_get_state(pearl_ctx* c, int d, double* phi, double* carbon, double* mu, pearl_scalars* scalars) {
+    int st = ctx_check(c, d);
+    if (st) return st;
+    st = read_scalars(c);
+    if (st) return st;
+    const int cur = c->hS[d].cur;
+    const size_t n = static_cast<size_t>(c->kp[d].n);
+    if (phi) HIP_CHECK(c, hipMemcpyAsync(phi, dom_phi(c, d, cur), 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (carbon) HIP_CHECK(c, hipMemcpyAsync(carbon, dom_c(c, d, cur), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (mu) HIP_CHECK(c, hipMemcpyAsync(mu, dom_mu(c, d, cur), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (scalars) *scalars = c->sc[d];
+    return PEARL_OK;
+}
+
+int pearl_set_state(pearl_ctx* c, const double* phi, const double* carbon, const double* mu, const pearl_scalars* s) {
+    return pearl_batch_set_state(c, 0, phi, carbon, mu, s);
+}
+
+int pearl_get_state(pearl_ctx* c, double* phi, double* carbon, double* mu, pearl_scalars* s) {
+    return pearl_batch_get_state(c, 0, phi, carbon, mu, s);
+}
+
+int pearl_batch_run(pearl_ctx* c, const double* stop_times, const long long* max_steps, long long* taken,
+                    long long* cg_total, int* cg_peak, pearl_step_info* last, int* status) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (!stop_times || !max_steps) return PEARL_ERR_ARGUMENT;
+    st = run_domains(c, stop_times, max_steps);
+    if (st) return st;
+    int first = PEARL_OK, first_d = -1;
+    for (int d = 0; d < c->ndom; ++d) {
+        const DevScal& h = c->hS[d];
+        if (taken) taken[d] = h.taken;
+        if (cg_total) cg_total[d] += h.cg_total;
+        if (cg_peak) cg_peak[d] = std::max(cg_peak[d], h.cg_peak);
+        if (last) info_of(h, &last[d]);
+        const int sd = status_of(c, d);
+        if (status) status[d] = sd;
+        if (sd && first_d < 0) {
+            first = sd;
+            first_d = d;
+        }
+    }
+    if (first_d >= 0) (void)status_of(c, first_d);  // the message of the first failing domain
+    return first;
+}
+
+int pearl_run(pearl_ctx* c, double stop_time, long long max_steps, long long* taken, long long* cg_total, int* cg_peak,
+              pearl_step_info* last) {
+    int status = 0;
+    pearl_step_info info{};
+    long long t = 0, cgt = cg_total ? *cg_total : 0;
+    int cgp = cg_peak ? *cg_peak : 0;
+    const int st = pearl_batch_run(c, &stop_time, &max_steps, &t, &cgt, &cgp, &info, &status);
     if (taken) *taken = t;
+    if (cg_total) *cg_total = cgt;
+    if (cg_peak) *cg_peak = cgp;
     if (last) *last = info;
     return st;
 }
 
+int pearl_advance(pearl_ctx* c, double stop_time, pearl_step_info* info) {
+    if (!c || !info) return PEARL_ERR_ARGUMENT;
+    long long taken = 0;
+    std::memset(info, 0, sizeof(*info));
+    return pearl_run(c, stop_time, 1, &taken, nullptr, nullptr, info);
+}
+
 int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double target, double c_gamma, int* shifted,
                       double* front) {
-    if (!c || !shifted) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (!shifted) return PEARL_ERR_ARGUMENT;
     *shifted = 0;
     if (front) *front = -1;
     if (!moving_window) return PEARL_OK;
-    const KP& P = c->kp;
-    {
-        ProfScope ps(c, PEARL_KERNEL_WINDOW, c->n);
-        PEARL_DISPATCH(c, k_front, grid_of(c), P, cur_phi(c), c->part, c->S);
-    }
-    int st = sync_scalars(c);
+    const KP& P = c->kp[0];
+    PEARL_LAUNCH(c, k_front, dim3(P.nb), c->B, 0);
+    st = read_scalars(c);
     if (st) return st;
-    const double fr = c->hS->front;
+    const double fr = c->hS[0].front;
     if (front) *front = fr;
     if (fr < trigger) return PEARL_OK;
     const int rows = static_cast<int>(std::floor((fr - target) / P.dy));
@@ -660,111 +649,163 @@ int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double ta
     if (scan_rows > 0) {
         const int m = scan_rows * P.nx;
         const int nbs = (m + TPB * c->k - 1) / (TPB * c->k);
-        {
-            ProfScope ps(c, PEARL_KERNEL_WINDOW, m);
-            PEARL_DISPATCH(c, k_strip, dim3(nbs), P, cur_phi(c), cur_c(c), m, nbs, c->part, c->S);
-        }
-        st = sync_scalars(c);
+        PEARL_LAUNCH(c, k_strip, dim3(nbs), c->B, 0, m, nbs);
+        st = read_scalars(c);
         if (st) return st;
-        if (c->hS->gmax > 1e-6) return fail(c, PEARL_ERR_UNSAFE_SHIFT, "unsafe shift: removed strip contains gamma");
-        removed = c->hS->removed;
+        if (c->hS[0].gmax > 1e-6) return fail(c, PEARL_ERR_UNSAFE_SHIFT, "unsafe shift: removed strip contains gamma");
+        removed = c->hS[0].removed;
     }
     if (rows <= 0 || rows >= P.ny || c_gamma < 0 || c_gamma > 1) return fail(c, PEARL_ERR_INVALID_SHIFT, "invalid shift");
     const double added = rows * P.nx * c_gamma * P.dx * P.dy;
-    {
-        ProfScope ps(c, PEARL_KERNEL_WINDOW, c->n);
-        PEARL_DISPATCH(c, k_shift, grid_of(c), P, cur_phi(c), cur_c(c), trial_phi(c), trial_c(c), trial_mu(c), rows,
-                       c_gamma, c->S);
-    }
-    st = sync_scalars(c);
+    PEARL_LAUNCH(c, k_shift, dim3(P.nb), c->B, 0, rows, c_gamma);
+    st = read_scalars(c);
     if (st) return st;
-    if (c->hS->err) {
-        const int f = c->hS->err;
-        return err_from_flags(c, f);
-    }
-    c->cur = 1 - c->cur;
-    c->sc.shift_distance += rows * P.dy;
-    c->sc.shifts++;
-    c->sc.carbon_added += added;
-    c->sc.carbon_removed += removed;
-    c->e0_valid = false;
-    c->m0_valid = false;
+    DevScal& h = c->hS[0];
+    if (h.err & ERRF_MIXTURE) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+    if (h.err & ERRF_NONFINITE_CARBON) return fail(c, PEARL_ERR_NONFINITE_CARBON, "Nonfinite total carbon");
+    h.cur ^= 1;
+    h.e0_valid = 0;
+    h.m0_valid = 0;
+    st = write_scalars(c);
+    if (st) return st;
+    c->sc[0].shift_distance += rows * P.dy;
+    c->sc[0].shifts++;
+    c->sc[0].carbon_added += added;
+    c->sc[0].carbon_removed += removed;
     *shifted = 1;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return PEARL_OK;
 }
 
 int pearl_diagnostics(pearl_ctx* c, pearl_diag* out) {
-    if (!c || !out) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    {
-        ProfScope ps(c, PEARL_KERNEL_STATE_REDUCE, c->n);
-        PEARL_DISPATCH(c, k_diagnostics, grid_of(c), c->kp, cur_phi(c), cur_c(c), cur_mu(c), c->part, c->S);
-    }
-    int st = sync_scalars(c);
+    int st = ctx_check(c, 0);
     if (st) return st;
-    if (c->hS->err & ERRF_MIXTURE) return err_from_flags(c, ERRF_MIXTURE);
-    out->c_min = c->hS->dmin_c;
-    out->c_max = c->hS->dmax_c;
-    out->mu_min = c->hS->dmin_mu;
-    out->mu_max = c->hS->dmax_mu;
-    out->phi_sum_error = c->hS->dsumerr;
-    out->mass = c->hS->dmass * c->kp.dx * c->kp.dy;
-    out->energy = c->hS->denergy * c->kp.dx * c->kp.dy;
+    if (!out) return PEARL_ERR_ARGUMENT;
+    const KP& P = c->kp[0];
+    PEARL_LAUNCH(c, k_diagnostics, dim3(P.nb), c->B, 0);
+    st = read_scalars(c);
+    if (st) return st;
+    const DevScal& h = c->hS[0];
+    if (!c->prm[0].phase_only && (h.err & ERRF_MIXTURE_STATE))
+        return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+    out->c_min = h.dmin_c;
+    out->c_max = h.dmax_c;
+    out->mu_min = h.dmin_mu;
+    out->mu_max = h.dmax_mu;
+    out->phi_sum_error = h.dsumerr;
+    out->mass = h.dmass * P.dx * P.dy;
+    out->energy = h.denergy * P.dx * P.dy;
     return PEARL_OK;
 }
 
 int pearl_propose_phi(pearl_ctx* c, double dt, double* phi_out, double* max_change) {
-    if (!c || !max_change) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    int st = propose(c, dt, max_change);
+    int st = ctx_check(c, 0);
     if (st) return st;
+    if (!max_change) return PEARL_ERR_ARGUMENT;
+    st = read_scalars(c);
+    if (st) return st;
+    DevScal& h = c->hS[0];
+    h.phase = PH_PROPOSE;
+    h.need_state = 0;
+    h.dt = dt;
+    h.status = ST_OK;
+    st = write_scalars(c);
+    if (st) return st;
+    PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
+    st = read_scalars(c);
+    if (st) return st;
+    const int status = h.status;
+    *max_change = h.change;
+    h.phase = PH_IDLE;
+    h.status = ST_OK;
+    st = write_scalars(c);
+    if (st) return st;
+    if (status == ST_PROJECTION) return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
     if (phi_out) {
-        HIP_CHECK(c, hipMemcpyAsync(phi_out, trial_phi(c), 3 * static_cast<size_t>(c->n) * sizeof(double),
+        HIP_CHECK(c, hipMemcpyAsync(phi_out, dom_phi(c, 0, h.cur ^ 1), 3 * static_cast<size_t>(c->kp[0].n) * sizeof(double),
                                     hipMemcpyDeviceToHost, c->stream));
-        HIP_CHECK(c, hipStreamSynchronize(c->stream));
     }
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return PEARL_OK;
 }
 
 int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, double* carbon_out, double* mu_out,
                              pearl_diffusion_info* info) {
-    if (!c || !phi_trial || !info) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    const size_t n = static_cast<size_t>(c->n);
-    HIP_CHECK(c, hipMemcpyAsync(trial_phi(c), phi_trial, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    DiffOut d;
-    int st = diffusion(c, dt, d);
+    int st = ctx_check(c, 0);
     if (st) return st;
-    info->ok = d.ok;
-    info->it = d.it;
-    info->residual = d.residual;
-    info->closure = d.closure;
-    const double* srcc = d.finalized ? trial_c(c) : cur_c(c);
-    const double* srcm = d.finalized ? trial_mu(c) : cur_mu(c);
-    if (carbon_out) HIP_CHECK(c, hipMemcpyAsync(carbon_out, srcc, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (mu_out) HIP_CHECK(c, hipMemcpyAsync(mu_out, srcm, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (!phi_trial || !info || c->prm[0].phase_only) return PEARL_ERR_ARGUMENT;
+    st = read_scalars(c);
+    if (st) return st;
+    DevScal& h = c->hS[0];
+    const size_t n = static_cast<size_t>(c->kp[0].n);
+    HIP_CHECK(c, hipMemcpyAsync(dom_phi(c, 0, h.cur ^ 1), phi_trial, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    h.phase = PH_SOLVE;
+    h.live = 1;
+    h.cg_state = CG_IDLE;
+    h.cg_it = 0;
+    h.cg_res = 0;
+    h.closure = 0;
+    h.pad = 0;
+    h.dt = dt;
+    h.status = ST_OK;
+    h.err = 0;
+    st = write_scalars(c);
+    if (st) return st;
+    // rounds without begin/decide until the solve leaves PH_SOLVE
+    for (int guard = 0; guard < 100000; ++guard) {
+        if (c->prof) {
+            hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
+            enqueue_round(c, c->cg_slots, true, false, false);
+        } else {
+            enqueue_round(c, c->cg_slots, false, false, false);
+        }
+        st = read_scalars(c);
+        if (st) return st;
+        if (h.phase != PH_SOLVE) break;
+    }
+    if (h.status != ST_OK) {
+        const int code = status_of(c, 0);
+        h.phase = PH_IDLE;
+        h.status = ST_OK;
+        (void)write_scalars(c);
+        return code;
+    }
+    const bool finalized = h.cg_state == CG_OK;
+    info->it = h.cg_it;
+    info->residual = h.cg_res;
+    info->closure = finalized ? h.closure : 0;
+    info->ok = finalized && info->closure <= c->prm[0].closure_tol;
+    h.phase = PH_IDLE;
+    st = write_scalars(c);
+    if (st) return st;
+    const int src = finalized ? (h.cur ^ 1) : h.cur;
+    if (carbon_out) HIP_CHECK(c, hipMemcpyAsync(carbon_out, dom_c(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (mu_out) HIP_CHECK(c, hipMemcpyAsync(mu_out, dom_mu(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return PEARL_OK;
 }
 
 int pearl_reduced_energy(pearl_ctx* c, double* energy) {
-    if (!c || !energy) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    return state_reduce(c, nullptr, nullptr, energy, !c->prm.phase_only);
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (!energy) return PEARL_ERR_ARGUMENT;
+    return state_direct(c, 0, !c->prm[0].phase_only, nullptr, nullptr, energy);
 }
 
 int pearl_total_carbon(pearl_ctx* c, double* mass) {
-    if (!c || !mass) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    return state_reduce(c, nullptr, mass, nullptr, false);
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (!mass) return PEARL_ERR_ARGUMENT;
+    return state_direct(c, 0, false, nullptr, mass, nullptr);
 }
 
 int pearl_valid_state(pearl_ctx* c, int* valid) {
-    if (!c || !valid) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    bool v = false;
-    int st = state_reduce(c, &v, nullptr, nullptr, false);
-    *valid = v ? 1 : 0;
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (!valid) return PEARL_ERR_ARGUMENT;
+    double v = 0;
+    st = state_direct(c, 0, false, &v, nullptr, nullptr);
+    *valid = v > 0.5 ? 1 : 0;
     return st;
 }
 
@@ -776,10 +817,10 @@ int pearl_get_layout(const pearl_ctx* c, int* tpb, int* cpt) {
 }
 
 int pearl_profile(pearl_ctx* c, int enable) {
-    if (!c) return PEARL_ERR_ARGUMENT;
-    (void)hipSetDevice(c->device);
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    harvest(c);
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    st = read_scalars(c);
+    if (st) return st;
     c->prof = enable != 0;
     for (int i = 0; i < PEARL_KERNEL_COUNT; ++i) {
         c->launches[i] = 0;
@@ -792,8 +833,8 @@ int pearl_profile(pearl_ctx* c, int enable) {
 int pearl_kernel_stats(pearl_ctx* c, int kernel, long long* launches, double* total_ms, double* bytes) {
     if (!c || kernel < 0 || kernel >= PEARL_KERNEL_COUNT) return PEARL_ERR_ARGUMENT;
     (void)hipSetDevice(c->device);
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    harvest(c);
+    const int st = read_scalars(c);
+    if (st) return st;
     if (launches) *launches = c->launches[kernel];
     if (total_ms) *total_ms = c->ms[kernel];
     if (bytes) *bytes = c->bytes[kernel];
@@ -807,9 +848,15 @@ const char* pearl_kernel_name(int kernel) {
 
 int pearl_counters(const pearl_ctx* c, long long* attempts, long long* rejected, long long* cg) {
     if (!c) return PEARL_ERR_ARGUMENT;
-    if (attempts) *attempts = c->attempts;
-    if (rejected) *rejected = c->rejected_attempts;
-    if (cg) *cg = c->cg_iterations;
+    long long a = 0, r = 0, g = 0;
+    for (int d = 0; d < c->ndom; ++d) {
+        a += c->hS[d].attempts;
+        r += c->hS[d].rejected_attempts;
+        g += c->hS[d].cg_all;
+    }
+    if (attempts) *attempts = a;
+    if (rejected) *rejected = r;
+    if (cg) *cg = g;
     return PEARL_OK;
 }
 

@@ -1,5 +1,6 @@
-// Device-side pieces of the FP64 per-step solver: kernel parameter block,
-// pointwise physics, the blocked reduction, and the kernels.
+// Device side of the per-step solver: per-domain parameter blocks, the
+// device-resident step controller, pointwise physics, the blocked
+// reduction, and the kernels.
 //
 // Each per-cell expression keeps the reference's operation order and
 // rounding (pearl_serial.cpp lines cited per function); the library is built
@@ -7,19 +8,33 @@
 // calls std::fma.  Loop-invariant quotients that the reference recomputes
 // per call (1/(2A), -0.25/A, 16 sigma/(pi^2 eps), ...) are evaluated once on
 // the host with the same expression, which yields the same bits.
+//
+// Control.  advance() (:1119-1174) runs on the device as a small state
+// machine per domain (DevScal::phase).  The host enqueues fixed "rounds" of
+// kernels -- begin, state reductions, propose, setup, residual, PCG slots,
+// finalize, decide -- and every kernel acts only on the domains whose phase
+// and PCG state call for it.  A round therefore has no host decision in it:
+// it can be replayed from a captured graph, and one launch serves any number
+// of independent domains of different sizes.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace pearl_gpu {
 
-constexpr int TPB = 256;  // threads per block of every cell kernel
+constexpr int TPB = 256;        // threads per block of every cell kernel
+constexpr int MAX_SLOTS = 64;   // per-launch work records (profiling)
+constexpr int NKIND = 16;       // per-domain run counters
 
-// Kernel parameter block (by value).  All derived constants are formed on
-// the host exactly as the reference forms them.
+// Per-domain parameter block.  Geometry of the domain inside the packed
+// arena, then the model constants, all derived on the host exactly as the
+// reference forms them.
 struct KP {
-    int nx, ny, n;          // grid; n = nx*ny cells (< 2^31)
-    int nb;                 // blocks of the cell kernels
+    int nx, ny, n;          // grid; n = nx*ny cells
+    int nb;                 // blocks of this domain in a cell-kernel grid
+    int boff;               // first block of this domain in the grid
+    long long off;          // first cell of this domain in every plane
+    long long poff;         // start of this domain's reduction partials
     double dx, dy, dx2, dy2;  // dx2 = dx*dx (the reference's d.dx*d.dx)
     double A[3], B[3], C[3], D[3];
     double chi[3];          // 1.0/(2.0*A)        Parabola::chi   :84
@@ -28,55 +43,61 @@ struct KP {
     double obs, tr;         // 16*sigma/(pi*pi*eps), triple/eps   :964
     double face_coef;       // 2*eps*sigma                        :981
     double eps_sigma;       // eps*sigma                          :816
-    double eps, Vm;
+    double eps, sigma, Vm;
     double tau[3], tau_min;
     double psiref[3];       // psi_a(mu_ref) - psi_gamma(mu_ref), psiref[2]=0  :797
     double mu_ref;
-    double max_phase_change, cg_rtol, cg_atol;  // controls used by device-side decisions
-    int cg_max;
-    int phase_only;
+    // Controls (:767-773)
+    double dt_max, dt_min, max_phase_change, cg_rtol, cg_atol, closure_tol, energy_rtol, energy_atol;
+    double mass_floor;      // gx*gy*dx*dy*1e-6                         :1152
+    double energy_scale;    // sigma*max(gx*dx, gy*dy)                  :1158
+    int cg_max, max_retries, check_energy, phase_only;
 };
 
-// Device-resident scalars written by the reduction finishers.
-struct DevScal {
-    double change;                        // propose_phi max |dphi|
-    double rhs2, rz, rr, pAp, alpha, beta;  // PCG
-    double tol, cg_res;                   // PCG tolerance (:1044) and info.residual
-    double closure, valid, mass, energy;  // finalize + acceptance
-    double e_valid, e_mass, e_energy;     // state reductions of the current state
-    double front, gmax, removed;          // moving window
-    double dmin_c, dmax_c, dmin_mu, dmax_mu, dsumerr, dmass, denergy;  // diagnostics
-    int live;                             // attempt still running (phase increment passed)
-    int cg_state;                         // CG_* below
-    int cg_it;                            // info.it (:1048)
-    int restart;                          // next direction is p = z (first iteration, restart)
-    int pidx;                             // which p buffer holds the current direction
-    int err;                              // ERRF_* bits
-    unsigned counter;                     // last-block counter
-    int pad;
-    int kind[64];                         // per-launch record of the PCG kernels (profiling)
-};
-
-enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4, ERRF_MIXTURE_STATE = 8 };
-
+// Controller phases of one domain.
+enum { PH_IDLE = 0, PH_START = 1, PH_PROPOSE = 2, PH_SOLVE = 3, PH_DECIDE = 4 };
 // PCG states (implicit_diffusion :1047-1080).  CG_ITER: take a step;
 // CG_TRUE: recompute the true residual (:1053); CG_OK / CG_FAIL: done;
-// CG_IDLE: no solve in this attempt.
+// CG_IDLE: not started.
 enum { CG_IDLE = 0, CG_ITER = 1, CG_TRUE = 2, CG_OK = 3, CG_FAIL = 4 };
-enum { KIND_NOOP = 0, KIND_ITER = 1, KIND_TRUE = 2 };
+// Device error codes (the C ABI's PEARL_ERR_* values).
+enum { ST_OK = 0, ST_INVALID_TIME = 1, ST_DT_MIN = 2, ST_RETRY_LIMIT = 3, ST_MIXTURE = 4, ST_PROJECTION = 5 };
+enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4, ERRF_MIXTURE_STATE = 8 };
+// Kernels counted per domain (work actually done).
+enum { RUN_PROPOSE = 0, RUN_SETUP, RUN_RESIDUAL0, RUN_CG_APPLY, RUN_CG_UPDATE, RUN_TRUE_RESIDUAL, RUN_FINALIZE,
+       RUN_STATE, RUN_COUNT };
 
-// The loop head of implicit_diffusion (:1047-1067) for iteration `it` with
-// the current r.r: info.it, info.residual, then the convergence and
-// iteration-limit branches.  Runs in one thread.
-__device__ __forceinline__ void cg_head(const int cg_max, DevScal* S, int it) {
-    const double res = __builtin_sqrt(S->rr);
-    S->cg_it = it;
-    S->cg_res = res;
-    if (!__builtin_isfinite(res)) S->cg_state = CG_FAIL;
-    else if (res <= S->tol) S->cg_state = CG_TRUE;
-    else if (it == cg_max) S->cg_state = CG_FAIL;
-    else S->cg_state = CG_ITER;
-}
+// Device-resident state of one domain: controller, step scalars, the
+// results of the reductions and the StepInfo of the last accepted step.
+struct DevScal {
+    // controller
+    int phase, status, why, attempt;
+    int cur;               // which state buffer is current
+    int need_state;        // e0/mass0 must be computed first
+    int e0_valid, m0_valid;
+    long long step, rejected, target_steps, taken;
+    double time, dt_state, stop_time;  // State::time, State::dt; stop_time of advance
+    double dt;             // dt of the current attempt
+    double e0, m0;
+    // attempt
+    int live;              // the phase increment passed (:1131)
+    int cg_state, cg_it, restart, pidx;
+    int err;               // ERRF_* bits
+    unsigned counter;      // last-block counter
+    double change;
+    double rhs2, rz, rr, pAp, alpha, beta, tol, cg_res;
+    double closure, valid, mass, energy;
+    double e_valid, e_mass, e_energy;
+    double front, gmax, removed;
+    double dmin_c, dmax_c, dmin_mu, dmax_mu, dsumerr, dmass, denergy;
+    // StepInfo of the last accepted step
+    double i_dt, i_change, i_residual, i_closure, i_energy_change;
+    int i_cg, i_retries;
+    // counters
+    long long attempts, rejected_attempts, cg_all, cg_total;
+    int cg_peak, pad;
+    unsigned runs[NKIND];
+};
 
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }  // std::min
@@ -85,10 +106,10 @@ __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b
 // Blocked reduction.  OPS packs 2 bits per quantity: 0 sum, 1 max, 2 min.
 // Stage 1 (caller): each thread folds its cells into v[q] in cell order.
 // Stage 2: 64-lane xor-halving tree, then (w0+w1)+(w2+w3) per block; block
-// partials to part[q*nb+b].  The last block to arrive (atomic counter)
-// reduces the partials in the same shape: thread t folds partials t, t+256,
-// ... then the same tree.  Returns true in thread 0 of that block, with the
-// final values in v.
+// partials to part[q*nb+b].  The last block of the domain to arrive (atomic
+// counter) reduces the partials in the same shape: thread t folds partials
+// t, t+256, ... then the same tree.  Returns true in thread 0 of that
+// block, with the final values in v.
 // ---------------------------------------------------------------------------
 template <unsigned OPS>
 __device__ __forceinline__ int op_of(int q) { return (OPS >> (2 * q)) & 3; }
@@ -112,7 +133,7 @@ __device__ __forceinline__ void wave_tree(double (&v)[NQ]) {
 }
 
 template <int NQ, unsigned OPS>
-__device__ bool block_reduce(double (&v)[NQ], double* __restrict__ part, unsigned* counter, int nb) {
+__device__ bool block_reduce(double (&v)[NQ], double* __restrict__ part, unsigned* counter, int nb, int lb) {
     __shared__ double sh[4][NQ];
     __shared__ int is_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -126,7 +147,7 @@ __device__ bool block_reduce(double (&v)[NQ], double* __restrict__ part, unsigne
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int op = op_of<OPS>(q);
-            part[q * nb + blockIdx.x] = comb(op, comb(op, sh[0][q], sh[1][q]), comb(op, sh[2][q], sh[3][q]));
+            part[q * nb + lb] = comb(op, comb(op, sh[0][q], sh[1][q]), comb(op, sh[2][q], sh[3][q]));
         }
         __threadfence();
         const unsigned prev = atomicAdd(counter, 1u);
@@ -329,23 +350,161 @@ __device__ __forceinline__ void accept_cell(const KP& P, const double* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Kernels.  KK = cells per thread; block b owns cells [b*256*KK, (b+1)*256*KK).
+// Packed arena and domain lookup
 // ---------------------------------------------------------------------------
-#define PEARL_CELL_LOOP(KK)                                                   \
+struct Bufs {
+    double* phi[2];   // domain d: phi[b] + 3*off + a*n + i
+    double* c[2];
+    double* mu[2];
+    double *chi, *icpt, *K, *rhs, *x, *diag, *r, *z, *p0, *p1, *Ap;
+    double* part;     // reduction partials, domain d at part + poff
+    const KP* kp;
+    DevScal* S;
+    const int* blk2dom;
+    int* kinds;       // per-launch work records (profiling; may be unused)
+    int ndom;
+};
+
+struct Where {
+    int d, lb;
+};
+
+// Domain and block-in-domain of this block: a round grid covers all
+// domains; a direct launch (only_d >= 0) covers one domain.
+__device__ __forceinline__ Where where(const Bufs& B, int only_d) {
+    if (only_d >= 0) return {only_d, static_cast<int>(blockIdx.x)};
+    const int d = B.blk2dom[blockIdx.x];
+    return {d, static_cast<int>(blockIdx.x) - B.kp[d].boff};
+}
+
+__device__ __forceinline__ void record(const Bufs& B, int slot, int lb, int kind) {
+    if (slot >= 0 && lb == 0 && threadIdx.x == 0) B.kinds[slot] = kind;
+}
+
+#define PEARL_CELL_LOOP(KK, P, lb)                                            \
     _Pragma("unroll") for (int j_ = 0; j_ < (KK); ++j_) {                     \
-        const int i = (blockIdx.x * (KK) + j_) * TPB + threadIdx.x;          \
-        if (i >= P.n) break;
+        const int i = ((lb) * (KK) + j_) * TPB + static_cast<int>(threadIdx.x); \
+        if (i >= (P).n) break;
 
 #define PEARL_CELL_LOOP_END }
 
-// P1: propose_phi :961-1002.
+// Loop head of implicit_diffusion (:1047-1067) for iteration `it` with the
+// current r.r: info.it, info.residual, then the convergence and
+// iteration-limit branches.  One thread.
+__device__ __forceinline__ void cg_head(const KP& P, DevScal* S, int it) {
+    const double res = __builtin_sqrt(S->rr);
+    S->cg_it = it;
+    S->cg_res = res;
+    if (!__builtin_isfinite(res)) S->cg_state = CG_FAIL;
+    else if (res <= S->tol) S->cg_state = CG_TRUE;
+    else if (it == P.cg_max) S->cg_state = CG_FAIL;
+    else S->cg_state = CG_ITER;
+    if (S->cg_state == CG_FAIL) S->phase = PH_DECIDE;
+}
+
+// ---------------------------------------------------------------------------
+// Round kernels
+// ---------------------------------------------------------------------------
+
+// Start of a step (:1120-1123) and reset of the per-attempt flags.  One
+// thread per domain.
+__global__ void k_begin(Bufs B) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= B.ndom) return;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (S->phase == PH_START) {
+        double dt = S->dt_state;
+        if (P.dt_max < dt) dt = P.dt_max;
+        if (S->stop_time - S->time < dt) dt = S->stop_time - S->time;
+        if (!(dt > 0) || !__builtin_isfinite(dt)) {
+            S->status = ST_INVALID_TIME;
+            S->phase = PH_IDLE;
+            return;
+        }
+        S->dt = dt;
+        S->attempt = 0;
+        S->phase = PH_PROPOSE;
+        S->need_state = ((P.check_energy && !S->e0_valid) || (!P.phase_only && !S->m0_valid)) ? 1 : 0;
+    }
+    if (S->phase == PH_PROPOSE) {
+        S->live = 0;
+        S->cg_state = CG_IDLE;
+        S->cg_it = 0;
+        S->cg_res = 0;
+        S->closure = 0;
+        S->restart = 1;
+        S->pidx = 0;
+        S->pad = 0;  // "stepped": k_cg_a took a step that k_cg_b must complete
+    }
+}
+
+// valid_state / total_carbon / reduced_energy of one cell of a state
+// (:1096-1116, :784-788, :793-821).
+__device__ __forceinline__ void state_cell(const KP& P, const double* phi, const double* c, const double* mu, int i,
+                                           bool& good, double& mass, double& energy, bool& mix_ok) {
+    const Nbr nb = neighbours(P, i);
+    double e;
+    const double ci = c[i];
+    accept_cell(P, phi, i, nb, ci, mu[i], good, e, mix_ok);
+    mass += ci;
+    energy += e;
+}
+
+// e0 and mass0 of a step (:1122-1123) when they are not cached.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_propose(KP P, const double* __restrict__ phi, const double* __restrict__ mu,
-                                                double* __restrict__ out, double dt, double* part, DevScal* S) {
+__global__ __launch_bounds__(TPB) void k_state(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (!(S->phase == PH_PROPOSE && S->need_state)) return;
+    record(B, slot, w.lb, 1);
+    const int cur = S->cur;
+    const double* phi = B.phi[cur] + 3 * P.off;
+    const double* c = B.c[cur] + P.off;
+    const double* mu = B.mu[cur] + P.off;
+    double v[4] = {1.0, 0.0, 0.0, 0.0};
+    bool good = true, mix_ok = true;
+    PEARL_CELL_LOOP(KK, P, w.lb)
+    state_cell(P, phi, c, mu, i, good, v[1], v[2], mix_ok);
+    PEARL_CELL_LOOP_END
+    v[0] = good ? 1.0 : 0.0;
+    v[3] = mix_ok ? 0.0 : 1.0;
+    if (block_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+        S->runs[RUN_STATE]++;
+        if (P.check_energy && v[3] > 0) {
+            S->status = ST_MIXTURE;
+            S->phase = PH_IDLE;
+            return;
+        }
+        if (P.check_energy) {
+            S->e0 = v[2] * P.dx * P.dy;
+            S->e0_valid = 1;
+        }
+        if (!P.phase_only) {
+            S->m0 = v[1] * P.dx * P.dy;
+            S->m0_valid = 1;
+        }
+        S->need_state = 0;
+    }
+}
+
+// P1: propose_phi :961-1002, then the phase-increment test (:1131).
+template <int KK>
+__global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (S->phase != PH_PROPOSE || S->need_state) return;
+    record(B, slot, w.lb, 1);
     const int n = P.n;
-    double v[1] = {0.0};
-    bool proj_fail = false;
-    PEARL_CELL_LOOP(KK)
+    const int cur = S->cur;
+    const double* __restrict__ phi = B.phi[cur] + 3 * P.off;
+    const double* __restrict__ mu = B.mu[cur] + P.off;
+    double* __restrict__ out = B.phi[cur ^ 1] + 3 * P.off;
+    const double dt = S->dt;
+    double v[2] = {0.0, 0.0};
+    PEARL_CELL_LOOP(KK, P, w.lb)
     const Nbr nb = neighbours(P, i);
     double p[3], force[3];
     load_phi(phi, n, i, p);
@@ -376,9 +535,9 @@ __global__ __launch_bounds__(TPB) void k_propose(KP P, const double* __restrict_
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = a + 1; b < 3; ++b) {
-                const double w = P.face_coef * (p[a] * q[b] - q[a] * p[b]) / dd;
-                force[a] += w * q[b];
-                force[b] -= w * q[a];
+                const double wv = P.face_coef * (p[a] * q[b] - q[a] * p[b]) / dd;
+                force[a] += wv * q[b];
+                force[b] -= wv * q[a];
             }
     }
     const double mean = (force[0] + force[1] + force[2]) / 3, scale = dt / (local_tau(P, p) * P.eps);
@@ -392,30 +551,45 @@ __global__ __launch_bounds__(TPB) void k_propose(KP P, const double* __restrict_
         continue;
     }
     double zp[3];
-    if (!project_simplex(z, zp)) proj_fail = true;
+    if (!project_simplex(z, zp)) v[1] = 1.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a * n + i] = zp[a];
     PEARL_CELL_LOOP_END
-    if (proj_fail) atomicOr(&S->err, ERRF_PROJECTION);
-    if (block_reduce<1, 1u>(v, part, &S->counter, P.nb)) {
+    if (block_reduce<2, (1u | (1u << 2))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+        S->runs[RUN_PROPOSE]++;
         S->change = v[0];
-        // :1131 -- the rest of the attempt runs only if the increment passes
+        if (v[1] > 0) {
+            S->status = ST_PROJECTION;
+            S->phase = PH_IDLE;
+            return;
+        }
         S->live = (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change) ? 1 : 0;
-        S->cg_state = CG_IDLE;
+        S->phase = S->live ? PH_SOLVE : PH_DECIDE;
     }
 }
 
 // P2: implicit_diffusion setup :1013-1020.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_setup(KP P, const double* __restrict__ phi_t, const double* __restrict__ c_old,
-                                              const double* __restrict__ mu_old, double* __restrict__ chi,
-                                              double* __restrict__ icpt, double* __restrict__ K,
-                                              double* __restrict__ rhs, double* __restrict__ x, DevScal* S) {
-    if (!S->live) return;
+__global__ __launch_bounds__(TPB) void k_setup(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
+    record(B, slot, w.lb, 1);
+    if (w.lb == 0 && threadIdx.x == 0) S->runs[RUN_SETUP]++;
+    const int cur = S->cur;
+    const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
+    const double* __restrict__ c_old = B.c[cur] + P.off;
+    const double* __restrict__ mu_old = B.mu[cur] + P.off;
+    double* __restrict__ chi = B.chi + P.off;
+    double* __restrict__ icpt = B.icpt + P.off;
+    double* __restrict__ K = B.K + P.off;
+    double* __restrict__ rhs = B.rhs + P.off;
+    double* __restrict__ x = B.x + P.off;
     bool bad = false;
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, w.lb)
     double p[3];
     load_phi(phi_t, P.n, i, p);
     Mix m;
@@ -434,19 +608,28 @@ __global__ __launch_bounds__(TPB) void k_setup(KP P, const double* __restrict__ 
 // (:1036-1045).  Sums rhs.rhs, r.z, r.r; the finisher forms the tolerance
 // and evaluates the loop head for iteration 0.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_residual0(KP P, const double* __restrict__ chi, const double* __restrict__ K,
-                                                  const double* __restrict__ rhs, const double* __restrict__ x,
-                                                  double* __restrict__ diag, double* __restrict__ r,
-                                                  double* __restrict__ z, double dt, double* part, DevScal* S) {
-    if (!S->live) return;
+__global__ __launch_bounds__(TPB) void k_residual0(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
+    record(B, slot, w.lb, 1);
+    const double dt = S->dt;
+    const double* __restrict__ chi = B.chi + P.off;
+    const double* __restrict__ K = B.K + P.off;
+    const double* __restrict__ rhs = B.rhs + P.off;
+    const double* __restrict__ x = B.x + P.off;
+    double* __restrict__ diag = B.diag + P.off;
+    double* __restrict__ r = B.r + P.off;
+    double* __restrict__ z = B.z + P.off;
     double v[3] = {0.0, 0.0, 0.0};
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, w.lb)
     const Nbr nb = neighbours(P, i);
-    double w[4];
-    face_weights(P, K, i, nb, w);
+    double wf[4];
+    face_weights(P, K, i, nb, wf);
     const double c = chi[i];
-    const double ax = apply_cell(P, x, c, dt, i, nb, w);
-    const double d = c + dt * (w[0] + w[1] + w[2] + w[3]);
+    const double ax = apply_cell(P, x, c, dt, i, nb, wf);
+    const double d = c + dt * (wf[0] + wf[1] + wf[2] + wf[3]);
     const double b = rhs[i];
     const double rr = b - ax;
     const double zz = rr / d;
@@ -457,13 +640,20 @@ __global__ __launch_bounds__(TPB) void k_residual0(KP P, const double* __restric
     v[1] += rr * zz;
     v[2] += rr * rr;
     PEARL_CELL_LOOP_END
-    if (block_reduce<3, 0u>(v, part, &S->counter, P.nb)) {
+    if (block_reduce<3, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+        S->runs[RUN_RESIDUAL0]++;
+        if (S->err & ERRF_MIXTURE) {
+            S->status = ST_MIXTURE;
+            S->phase = PH_IDLE;
+            return;
+        }
         S->rhs2 = v[0];
         S->rz = v[1];
         S->rr = v[2];
         S->tol = P.cg_atol + P.cg_rtol * __builtin_sqrt(v[0]);
         S->restart = 1;
-        cg_head(P.cg_max, S, 0);
+        S->pad = 0;
+        cg_head(P, S, 0);
     }
 }
 
@@ -473,62 +663,73 @@ __global__ __launch_bounds__(TPB) void k_residual0(KP P, const double* __restric
 // for breakdown (:1070) and forms the step length (:1071).  In state
 // CG_TRUE: the true residual r = rhs - A x with z = r/diag (:1053-1062);
 // the finisher accepts (:1056) or restarts from it (:1060-1065).
-// `slot` indexes the per-launch record used by the profiler.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_cg_a(KP P, const double* __restrict__ chi, const double* __restrict__ K,
-                                             const double* __restrict__ rhs, const double* __restrict__ x,
-                                             const double* __restrict__ diag, double* __restrict__ r,
-                                             double* __restrict__ z, double* __restrict__ p0,
-                                             double* __restrict__ p1, double* __restrict__ Ap, double dt, int slot,
-                                             double* part, DevScal* S) {
+__global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (S->phase != PH_SOLVE) return;
     const int state = S->cg_state;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && slot >= 0)
-        S->kind[slot] = state == CG_ITER ? KIND_ITER : (state == CG_TRUE ? KIND_TRUE : KIND_NOOP);
+    const double dt = S->dt;
+    const double* __restrict__ chi = B.chi + P.off;
+    const double* __restrict__ K = B.K + P.off;
+    double* __restrict__ z = B.z + P.off;
     if (state == CG_ITER) {
+        record(B, slot, w.lb, 1);
         const int pi = S->pidx;
-        const double* __restrict__ po = pi ? p1 : p0;
-        double* __restrict__ pn = pi ? p0 : p1;
+        const double* __restrict__ po = (pi ? B.p1 : B.p0) + P.off;
+        double* __restrict__ pn = (pi ? B.p0 : B.p1) + P.off;
+        double* __restrict__ Ap = B.Ap + P.off;
         const bool fresh = S->restart != 0;
         const double beta = S->beta;
         double v[1] = {0.0};
-        PEARL_CELL_LOOP(KK)
+        PEARL_CELL_LOOP(KK, P, w.lb)
         const Nbr nb = neighbours(P, i);
-        double w[4];
-        face_weights(P, K, i, nb, w);
+        double wf[4];
+        face_weights(P, K, i, nb, wf);
         const double pc = fresh ? z[i] : z[i] + beta * po[i];
         const double pw = fresh ? z[nb.w] : z[nb.w] + beta * po[nb.w];
         const double pe = fresh ? z[nb.e] : z[nb.e] + beta * po[nb.e];
         const double ps = fresh ? z[nb.s] : z[nb.s] + beta * po[nb.s];
-        const double pnn = fresh ? z[nb.nn] : z[nb.nn] + beta * po[nb.nn];
-        const double a = chi[i] * pc + dt * (w[0] * (pc - pw) + w[1] * (pc - pe) + w[2] * (pc - ps) + w[3] * (pc - pnn));
+        const double pq = fresh ? z[nb.nn] : z[nb.nn] + beta * po[nb.nn];
+        const double a = chi[i] * pc + dt * (wf[0] * (pc - pw) + wf[1] * (pc - pe) + wf[2] * (pc - ps) + wf[3] * (pc - pq));
         pn[i] = pc;
         Ap[i] = a;
         v[0] += pc * a;
         PEARL_CELL_LOOP_END
-        if (block_reduce<1, 0u>(v, part, &S->counter, P.nb)) {
+        if (block_reduce<1, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+            S->runs[RUN_CG_APPLY]++;
             const double pAp = v[0], rz = S->rz;
             S->pAp = pAp;
             if (!(pAp > 0) || !__builtin_isfinite(pAp) || !__builtin_isfinite(rz)) {
                 S->cg_state = CG_FAIL;
+                S->phase = PH_DECIDE;
             } else {
                 S->alpha = rz / pAp;
                 S->pidx = pi ^ 1;
+                S->pad = 1;
             }
         }
     } else if (state == CG_TRUE) {
+        record(B, slot, w.lb, 2);
+        const double* __restrict__ rhs = B.rhs + P.off;
+        const double* __restrict__ x = B.x + P.off;
+        const double* __restrict__ diag = B.diag + P.off;
+        double* __restrict__ r = B.r + P.off;
         double v[2] = {0.0, 0.0};
-        PEARL_CELL_LOOP(KK)
+        PEARL_CELL_LOOP(KK, P, w.lb)
         const Nbr nb = neighbours(P, i);
-        double w[4];
-        face_weights(P, K, i, nb, w);
-        const double rr = rhs[i] - apply_cell(P, x, chi[i], dt, i, nb, w);
+        double wf[4];
+        face_weights(P, K, i, nb, wf);
+        const double rr = rhs[i] - apply_cell(P, x, chi[i], dt, i, nb, wf);
         const double zz = rr / diag[i];
         r[i] = rr;
         z[i] = zz;
         v[0] += rr * rr;
         v[1] += rr * zz;
         PEARL_CELL_LOOP_END
-        if (block_reduce<2, 0u>(v, part, &S->counter, P.nb)) {
+        if (block_reduce<2, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+            S->runs[RUN_TRUE_RESIDUAL]++;
             S->rr = v[0];
             const double res = __builtin_sqrt(v[0]);
             S->cg_res = res;
@@ -537,27 +738,31 @@ __global__ __launch_bounds__(TPB) void k_cg_a(KP P, const double* __restrict__ c
             } else {
                 S->rz = v[1];
                 S->restart = 1;
-                cg_head(P.cg_max, S, S->cg_it + 1);
+                cg_head(P, S, S->cg_it + 1);
             }
         }
     }
 }
 
-// PCG kernel B (CG_ITER only): x += alpha p; r -= alpha Ap; z = r/diag; sums
-// r.z and r.r; the finisher forms beta (:1077) and evaluates the loop head
-// of the next iteration.
+// PCG kernel B, after a step of kernel A: x += alpha p; r -= alpha Ap;
+// z = r/diag; sums r.z and r.r; the finisher forms beta (:1077) and
+// evaluates the loop head of the next iteration.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_cg_b(KP P, double* __restrict__ x, double* __restrict__ r,
-                                             double* __restrict__ z, const double* __restrict__ p0,
-                                             const double* __restrict__ p1, const double* __restrict__ Ap,
-                                             const double* __restrict__ diag, int slot, double* part, DevScal* S) {
-    const int state = S->cg_state;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && slot >= 0) S->kind[slot] = state == CG_ITER ? KIND_ITER : KIND_NOOP;
-    if (state != CG_ITER) return;
-    const double* __restrict__ p = S->pidx ? p1 : p0;
+__global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_ITER && S->pad == 1)) return;
+    record(B, slot, w.lb, 1);
+    const double* __restrict__ p = (S->pidx ? B.p1 : B.p0) + P.off;
+    double* __restrict__ x = B.x + P.off;
+    double* __restrict__ r = B.r + P.off;
+    double* __restrict__ z = B.z + P.off;
+    const double* __restrict__ Ap = B.Ap + P.off;
+    const double* __restrict__ diag = B.diag + P.off;
     const double alpha = S->alpha;
     double v[2] = {0.0, 0.0};
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, w.lb)
     x[i] += alpha * p[i];
     const double rr = r[i] - alpha * Ap[i];
     const double zz = rr / diag[i];
@@ -566,35 +771,46 @@ __global__ __launch_bounds__(TPB) void k_cg_b(KP P, double* __restrict__ x, doub
     v[0] += rr * zz;
     v[1] += rr * rr;
     PEARL_CELL_LOOP_END
-    if (block_reduce<2, 0u>(v, part, &S->counter, P.nb)) {
+    if (block_reduce<2, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+        S->runs[RUN_CG_UPDATE]++;
         const double next = v[0];
         S->beta = next / S->rz;
         S->rz = next;
         S->rr = v[1];
         S->restart = 0;
-        cg_head(P.cg_max, S, S->cg_it + 1);
+        S->pad = 0;
+        cg_head(P, S, S->cg_it + 1);
     }
 }
 
 // F + A: flux-form carbon update, closure, mu reconstruction (:1082-1093),
 // fused with the acceptance sweep on the trial state: valid_state (:1096),
 // total_carbon (:784) and reduced_energy (:793).
-// Reductions: closure max, valid min, carbon sum, energy sum.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_finalize(KP P, const double* __restrict__ phi_t,
-                                                 const double* __restrict__ chi, const double* __restrict__ icpt,
-                                                 const double* __restrict__ K, const double* __restrict__ x,
-                                                 const double* __restrict__ c_old, double* __restrict__ c_new,
-                                                 double* __restrict__ mu_new, double dt, double* part, DevScal* S) {
-    if (S->cg_state != CG_OK) return;
+__global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
+    record(B, slot, w.lb, 1);
+    const int cur = S->cur;
+    const double dt = S->dt;
+    const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
+    const double* __restrict__ chi = B.chi + P.off;
+    const double* __restrict__ icpt = B.icpt + P.off;
+    const double* __restrict__ K = B.K + P.off;
+    const double* __restrict__ x = B.x + P.off;
+    const double* __restrict__ c_old = B.c[cur] + P.off;
+    double* __restrict__ c_new = B.c[cur ^ 1] + P.off;
+    double* __restrict__ mu_new = B.mu[cur ^ 1] + P.off;
     double v[4] = {0.0, 1.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, w.lb)
     const Nbr nb = neighbours(P, i);
-    double w[4];
-    face_weights(P, K, i, nb, w);
+    double wf[4];
+    face_weights(P, K, i, nb, wf);
     const double xi = x[i];
-    const double change = dt * (w[0] * (x[nb.w] - xi) + w[1] * (x[nb.e] - xi) + w[2] * (x[nb.s] - xi) + w[3] * (x[nb.nn] - xi));
+    const double change = dt * (wf[0] * (x[nb.w] - xi) + wf[1] * (x[nb.e] - xi) + wf[2] * (x[nb.s] - xi) + wf[3] * (x[nb.nn] - xi));
     const double cn = c_old[i] + change;
     const double ic = icpt[i], ch = chi[i];
     v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
@@ -607,25 +823,33 @@ __global__ __launch_bounds__(TPB) void k_finalize(KP P, const double* __restrict
     v[3] += e;
     PEARL_CELL_LOOP_END
     v[1] = good ? 1.0 : 0.0;
-    if (!mix_ok) atomicOr(&S->err, ERRF_MIXTURE);
-    if (block_reduce<4, (1u | (2u << 2))>(v, part, &S->counter, P.nb)) {
+    if (block_reduce<4, (1u | (2u << 2))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+        S->runs[RUN_FINALIZE]++;
         S->closure = v[0];
         S->valid = v[1];
         S->mass = v[2];
         S->energy = v[3];
+        S->phase = PH_DECIDE;
     }
 }
 
 // Phase-only acceptance (:1134-1136): carbon = composition(mu, mixture(phi')),
 // mu unchanged; then valid_state and reduced_energy on the trial state.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_phase_only(KP P, const double* __restrict__ phi_t,
-                                                   const double* __restrict__ mu_old, double* __restrict__ c_new,
-                                                   double* __restrict__ mu_new, double* part, DevScal* S) {
-    if (!S->live) return;
-    double v[4] = {0.0, 1.0, 0.0, 0.0};
+__global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
+    const Where w = where(B, -1);
+    const KP& P = B.kp[w.d];
+    DevScal* S = B.S + w.d;
+    if (!(S->phase == PH_SOLVE && P.phase_only)) return;
+    record(B, slot, w.lb, 1);
+    const int cur = S->cur;
+    const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
+    const double* __restrict__ mu_old = B.mu[cur] + P.off;
+    double* __restrict__ c_new = B.c[cur ^ 1] + P.off;
+    double* __restrict__ mu_new = B.mu[cur ^ 1] + P.off;
+    double v[5] = {0.0, 1.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, w.lb)
     const Nbr nb = neighbours(P, i);
     double p[3];
     load_phi(phi_t, P.n, i, p);
@@ -641,49 +865,152 @@ __global__ __launch_bounds__(TPB) void k_phase_only(KP P, const double* __restri
     v[3] += e;
     PEARL_CELL_LOOP_END
     v[1] = good ? 1.0 : 0.0;
-    if (!mix_ok) atomicOr(&S->err, ERRF_MIXTURE);
-    if (block_reduce<4, (1u | (2u << 2))>(v, part, &S->counter, P.nb)) {
-        S->closure = v[0];
+    v[4] = mix_ok ? 0.0 : 1.0;
+    if (block_reduce<5, (1u | (2u << 2) | (1u << 8))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
+        S->runs[RUN_FINALIZE]++;
+        if (v[4] > 0) {
+            S->status = ST_MIXTURE;
+            S->phase = PH_IDLE;
+            return;
+        }
+        S->closure = 0;
         S->valid = v[1];
         S->mass = v[2];
         S->energy = v[3];
+        S->phase = PH_DECIDE;
     }
 }
 
-// valid_state, total_carbon and reduced_energy of a state (e0, mass0 and the
-// single-function entry points).
+// Acceptance tests and the dt controller of advance (:1131-1173).  One
+// thread per domain.
+__global__ void k_decide(Bufs B) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= B.ndom) return;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (S->phase != PH_DECIDE) return;
+    bool good = S->live != 0;
+    int why = 0;  // phase increment
+    double residual = 0, closure = 0;
+    int it = 0;
+    if (good && !P.phase_only) {
+        it = S->cg_it;
+        residual = S->cg_res;
+        S->cg_all += it;
+        closure = S->cg_state == CG_OK ? S->closure : 0;
+        good = S->cg_state == CG_OK && closure <= P.closure_tol;
+        why = 1;  // linear/closure
+    }
+    const double mass = S->mass * P.dx * P.dy;
+    const double energy = S->energy * P.dx * P.dy;
+    if (good) {
+        good = S->valid > 0.5;
+        why = 2;  // composition/phase bounds
+    }
+    if (good && !P.phase_only) {
+        good = __builtin_fabs(mass - S->m0) <= 1e-11 * dmaxd(__builtin_fabs(S->m0), P.mass_floor);
+        why = 3;  // mass conservation
+    }
+    double ech = 0;
+    if (good && P.check_energy) {
+        ech = energy - S->e0;
+        const double tol = P.energy_atol + P.energy_rtol * dmaxd(__builtin_fabs(S->e0), P.energy_scale);
+        good = __builtin_isfinite(energy) && energy <= S->e0 + tol;
+        why = 4;  // free-energy increase
+    }
+    S->attempts++;
+    if (good) {
+        const double dt = S->dt;
+        S->cur ^= 1;
+        S->time = S->time + dt;
+        S->step = S->step + 1;
+        S->rejected = S->rejected + S->attempt;
+        S->dt_state = dmind(P.dt_max, dt * 1.1);
+        if (P.check_energy) {
+            S->e0 = energy;
+            S->e0_valid = 1;
+        }
+        if (!P.phase_only) {
+            S->m0 = mass;
+            S->m0_valid = 1;
+        }
+        S->i_dt = dt;
+        S->i_change = S->change;
+        S->i_residual = residual;
+        S->i_closure = closure;
+        S->i_energy_change = ech;
+        S->i_cg = it;
+        S->i_retries = S->attempt;
+        S->cg_total += it;
+        if (it > S->cg_peak) S->cg_peak = it;
+        S->taken++;
+        S->why = -1;
+        S->phase = (S->taken >= S->target_steps || !(S->time < S->stop_time)) ? PH_IDLE : PH_START;
+        return;
+    }
+    S->rejected_attempts++;
+    S->why = why;
+    S->dt = S->dt * .5;
+    if (S->dt < P.dt_min) {
+        S->status = ST_DT_MIN;
+        S->phase = PH_IDLE;
+    } else if (S->attempt >= P.max_retries) {
+        S->status = ST_RETRY_LIMIT;
+        S->phase = PH_IDLE;
+    } else {
+        S->attempt++;
+        S->phase = PH_PROPOSE;
+    }
+}
+
+__global__ void k_clear_kinds(int* kinds, int n) {
+    const int i = threadIdx.x;
+    if (i < n) kinds[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Direct (single-domain) kernels: state reductions, diagnostics, window.
+// ---------------------------------------------------------------------------
+
+// valid_state, total_carbon and reduced_energy of domain d's current state.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_state_reduce(KP P, const double* __restrict__ phi,
-                                                     const double* __restrict__ c, const double* __restrict__ mu,
-                                                     double* part, DevScal* S) {
-    double v[3] = {1.0, 0.0, 0.0};
+__global__ __launch_bounds__(TPB) void k_state_direct(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const int cur = S->cur;
+    const double* phi = B.phi[cur] + 3 * P.off;
+    const double* c = B.c[cur] + P.off;
+    const double* mu = B.mu[cur] + P.off;
+    double v[4] = {1.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK)
-    const Nbr nb = neighbours(P, i);
-    double e;
-    const double ci = c[i];
-    accept_cell(P, phi, i, nb, ci, mu[i], good, e, mix_ok);
-    v[1] += ci;
-    v[2] += e;
+    PEARL_CELL_LOOP(KK, P, lb)
+    state_cell(P, phi, c, mu, i, good, v[1], v[2], mix_ok);
     PEARL_CELL_LOOP_END
     v[0] = good ? 1.0 : 0.0;
-    if (!mix_ok) atomicOr(&S->err, ERRF_MIXTURE_STATE);
-    if (block_reduce<3, 2u>(v, part, &S->counter, P.nb)) {
+    v[3] = mix_ok ? 0.0 : 1.0;
+    if (block_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, &S->counter, P.nb, lb)) {
         S->e_valid = v[0];
         S->e_mass = v[1];
         S->e_energy = v[2];
+        S->err = v[3] > 0 ? (S->err | ERRF_MIXTURE_STATE) : (S->err & ~ERRF_MIXTURE_STATE);
     }
 }
 
 // Snapshot reductions (:2267-2282): min/max of carbon and mu, max phase-sum
 // error, carbon sum, energy.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_diagnostics(KP P, const double* __restrict__ phi,
-                                                    const double* __restrict__ c, const double* __restrict__ mu,
-                                                    double* part, DevScal* S) {
-    double v[7] = {1e300, -1e300, 1e300, -1e300, 0.0, 0.0, 0.0};
+__global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const int cur = S->cur;
+    const double* phi = B.phi[cur] + 3 * P.off;
+    const double* c = B.c[cur] + P.off;
+    const double* mu = B.mu[cur] + P.off;
+    double v[8] = {1e300, -1e300, 1e300, -1e300, 0.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, lb)
     const Nbr nb = neighbours(P, i);
     const double ci = c[i], mi = mu[i];
     v[0] = dmind(v[0], ci);
@@ -698,9 +1025,9 @@ __global__ __launch_bounds__(TPB) void k_diagnostics(KP P, const double* __restr
     v[5] += ci;
     v[6] += e;
     PEARL_CELL_LOOP_END
-    if (!mix_ok) atomicOr(&S->err, ERRF_MIXTURE);
-    constexpr unsigned OPS = 2u | (1u << 2) | (2u << 4) | (1u << 6) | (1u << 8);
-    if (block_reduce<7, OPS>(v, part, &S->counter, P.nb)) {
+    v[7] = mix_ok ? 0.0 : 1.0;
+    constexpr unsigned OPS = 2u | (1u << 2) | (2u << 4) | (1u << 6) | (1u << 8) | (1u << 14);
+    if (block_reduce<8, OPS>(v, B.part + P.poff, &S->counter, P.nb, lb)) {
         S->dmin_c = v[0];
         S->dmax_c = v[1];
         S->dmin_mu = v[2];
@@ -708,39 +1035,46 @@ __global__ __launch_bounds__(TPB) void k_diagnostics(KP P, const double* __restr
         S->dsumerr = v[4];
         S->dmass = v[5];
         S->denergy = v[6];
+        S->err = v[7] > 0 ? (S->err | ERRF_MIXTURE_STATE) : (S->err & ~ERRF_MIXTURE_STATE);
     }
 }
 
 // Moving-window front (:2057-2062): max over interior faces of the gamma=1/2
 // crossing height.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_front(KP P, const double* __restrict__ phi, double* part, DevScal* S) {
-    const double* g = phi + 2 * P.n;
+__global__ __launch_bounds__(TPB) void k_front(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const double* g = B.phi[S->cur] + 3 * P.off + 2 * static_cast<long long>(P.n);
     double v[1] = {-1.0};
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, lb)
     const int y = i / P.nx;
     if (y < P.ny - 1) {
         const double a = 1 - 2 * g[i], b = 1 - 2 * g[i + P.nx];
         if (a >= 0 && b < 0) v[0] = dmaxd(v[0], ((y + 1) - .5 + a / (a - b)) * P.dy);
     }
     PEARL_CELL_LOOP_END
-    if (block_reduce<1, 1u>(v, part, &S->counter, P.nb)) S->front = v[0];
+    if (block_reduce<1, 1u>(v, B.part + P.poff, &S->counter, P.nb, lb)) S->front = v[0];
 }
 
-// Max gamma and removed carbon (c*dx)*dy over the first m cells (:2028, :2067).
-// Launched with ceil(m/(256*KK)) blocks; nb is that count.
+// Max gamma and removed carbon (c*dx)*dy over the first m cells (:2028,
+// :2067).  Launched with nbs = ceil(m/(256*KK)) blocks.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_strip(KP P, const double* __restrict__ phi, const double* __restrict__ c,
-                                              int m, int nb, double* part, DevScal* S) {
-    const double* g = phi + 2 * P.n;
+__global__ __launch_bounds__(TPB) void k_strip(Bufs B, int d, int m, int nbs) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int cur = S->cur;
+    const double* g = B.phi[cur] + 3 * P.off + 2 * static_cast<long long>(P.n);
+    const double* c = B.c[cur] + P.off;
     double v[2] = {0.0, 0.0};
     _Pragma("unroll") for (int j_ = 0; j_ < KK; ++j_) {
-        const int i = (blockIdx.x * KK + j_) * TPB + threadIdx.x;
+        const int i = (static_cast<int>(blockIdx.x) * KK + j_) * TPB + static_cast<int>(threadIdx.x);
         if (i >= m) break;
         v[0] = dmaxd(v[0], g[i]);
         v[1] += c[i] * P.dx * P.dy;
     }
-    if (block_reduce<2, 1u>(v, part, &S->counter, nb)) {
+    if (block_reduce<2, 1u>(v, B.part + P.poff, &S->counter, nbs, blockIdx.x)) {
         S->gmax = v[0];
         S->removed = v[1];
     }
@@ -749,12 +1083,19 @@ __global__ __launch_bounds__(TPB) void k_strip(KP P, const double* __restrict__ 
 // shift_rows row move and gamma fill (:2029-2042) into the other buffer,
 // then close_mu (:789-792) on the new fields.
 template <int KK>
-__global__ __launch_bounds__(TPB) void k_shift(KP P, const double* __restrict__ phi_in, const double* __restrict__ c_in,
-                                              double* __restrict__ phi_out, double* __restrict__ c_out,
-                                              double* __restrict__ mu_out, int rows, double c_gamma, DevScal* S) {
+__global__ __launch_bounds__(TPB) void k_shift(Bufs B, int d, int rows, double c_gamma) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const int cur = S->cur;
     const int n = P.n;
+    const double* __restrict__ phi_in = B.phi[cur] + 3 * P.off;
+    const double* __restrict__ c_in = B.c[cur] + P.off;
+    double* __restrict__ phi_out = B.phi[cur ^ 1] + 3 * P.off;
+    double* __restrict__ c_out = B.c[cur ^ 1] + P.off;
+    double* __restrict__ mu_out = B.mu[cur ^ 1] + P.off;
     bool bad_mix = false, bad_c = false;
-    PEARL_CELL_LOOP(KK)
+    PEARL_CELL_LOOP(KK, P, lb)
     const int y = i / P.nx;
     double p[3], c;
     if (y + rows < P.ny) {
@@ -779,15 +1120,7 @@ __global__ __launch_bounds__(TPB) void k_shift(KP P, const double* __restrict__ 
     if (bad_c) atomicOr(&S->err, ERRF_NONFINITE_CARBON);
 }
 
-// Clears the error bits (a kernel rather than a memset so that the call is
-// safe inside captured graphs).
-__global__ void k_clear_flags(DevScal* S) { S->err = 0; }
-
-// Marks an attempt live without a phase proposal (implicit_diffusion on a
-// caller-supplied trial phi).
-__global__ void k_begin_solve(DevScal* S) {
-    S->live = 1;
-    S->cg_state = CG_IDLE;
-}
+// Clears error bits (a kernel, not a memset, so it is safe in graphs).
+__global__ void k_clear_err(Bufs B, int d) { B.S[d].err = 0; }
 
 }  // namespace pearl_gpu

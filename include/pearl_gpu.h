@@ -112,7 +112,9 @@ int pearl_advance(pearl_ctx* ctx, double stop_time, pearl_step_info* info);
 /* Up to max_steps accepted steps toward stop_time without host I/O between
  * them (the step loop :2296-2305 between two output times).  Stops early at
  * stop_time.  *taken receives the number of accepted steps; cg_total and
- * cg_peak accumulate as in the reference main loop. */
+ * cg_peak accumulate as in the reference main loop.  The steps run under
+ * device-side control: the host replays captured rounds of kernels and
+ * synchronises only to see whether the run has finished. */
 int pearl_run(pearl_ctx* ctx, double stop_time, long long max_steps, long long* taken,
               long long* cg_total, int* cg_peak, pearl_step_info* last);
 
@@ -167,8 +169,27 @@ int pearl_profile(pearl_ctx* ctx, int enable);
 int pearl_kernel_stats(pearl_ctx* ctx, int kernel, long long* launches, double* total_ms, double* bytes);
 const char* pearl_kernel_name(int kernel);
 
-/* Attempt/iteration counters since creation: attempts, rejected attempts,
- * total CG iterations over all attempts. */
+/* Batched ensemble: `count` independent domains of any sizes in one
+ * context (one packed HBM arena; one launch of each kernel serves all of
+ * them).  There is no reference counterpart; each domain's step is the
+ * reference's advance.  pearl_batch_run advances domain d until it has taken
+ * max_steps[d] accepted steps or reached stop_times[d]; the arrays taken,
+ * cg_total (accumulated), cg_peak, last and status have `count` entries and
+ * may be NULL.  Returns the first failing domain's status (message in
+ * pearl_last_error); the others keep running to their own targets.  The
+ * single-domain calls above act on domain 0. */
+int pearl_batch_create(int count, const pearl_params* params, const double* mu_references, int precision,
+                       int device, int cells_per_thread, pearl_ctx** out);
+int pearl_batch_size(const pearl_ctx* ctx, int* count);
+int pearl_batch_set_state(pearl_ctx* ctx, int domain, const double* phi, const double* carbon, const double* mu,
+                          const pearl_scalars* scalars);
+int pearl_batch_get_state(pearl_ctx* ctx, int domain, double* phi, double* carbon, double* mu,
+                          pearl_scalars* scalars);
+int pearl_batch_run(pearl_ctx* ctx, const double* stop_times, const long long* max_steps, long long* taken,
+                    long long* cg_total, int* cg_peak, pearl_step_info* last, int* status);
+
+/* Attempt/iteration counters since creation (summed over domains):
+ * attempts, rejected attempts, total CG iterations over all attempts. */
 int pearl_counters(const pearl_ctx* ctx, long long* attempts, long long* rejected_attempts,
                    long long* cg_iterations);
 

@@ -207,3 +207,46 @@ def test_full_smoke_trajectory_vs_reference(pkg):
     say("  max|dphi|", dphi, "max|dmu|", dmu, "max|dc|", np.abs(c - g["c1"]).max())
     assert dphi <= PHI_TOL
     assert dmu <= MU_TOL
+
+
+def test_batch_of_uneven_domains_matches_single_domains(pkg, O):
+    """Three domains of different sizes and physics in one context: each is
+    bitwise equal to its own oracle trajectory in the context's blocked
+    order."""
+    names = ["small_48x40", "window_alpha_32x40", "colony_256x128"]
+    gs = [load_golden(n) for n in names]
+    steps = [12, 5, 8]
+    say("batch of", names)
+    with pkg.Batch([g["pd"] for g in gs], [float(g["sc0"][9]) for g in gs]) as b:
+        k = b.layout()[1]
+        for d, g in enumerate(gs):
+            b.set_domain_state(d, g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        r = b.run_all([float(g["end_time"]) for g in gs], steps)
+        assert r["rc"] == 0, r
+        assert r["taken"] == steps
+        states = [b.get_domain_state(d) for d in range(len(gs))]
+    for d, g in enumerate(gs):
+        P = blocked_params(O, g, k)
+        S = O.OracleState(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        for _ in range(steps[d]):
+            rc, _ = O.advance(P, S, float(g["sc0"][9]), float(g["end_time"]))
+            assert rc == 0
+        phi, c, mu, sc = states[d]
+        assert np.array_equal(phi, S.phi) and np.array_equal(c, S.carbon) and np.array_equal(mu, S.mu), names[d]
+        assert sc.to_list() == S.scalars(), names[d]
+
+
+def test_profiled_and_graph_runs_agree(pkg):
+    """Eager (profiled) rounds and captured-graph rounds give the same bits."""
+    g = load_golden("small_48x40")
+    out = []
+    for prof in (False, True):
+        with make(pkg, g) as s:
+            s.profile(prof)
+            s.run(float(g["end_time"]), 15)
+            out.append(s.get_state())
+            if prof:
+                st = s.kernel_stats()
+                assert st["propose_phi"]["launches"] >= 15 and st["cg_apply"]["bytes"] > 0
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(a, b)
