@@ -66,42 +66,38 @@ def build_workload(pkg, nx, ny, pairs, strip_rows, seed, undercooling, tmpdir):
     return path, cfg, phi, c, mu
 
 
-def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds):
-    """Time the reference (or the oracle) on the same state for ~`seconds`."""
+def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
+    """Time the reference (built from source) on the same state for about
+    `seconds`, in a child process (the reference library carries its own
+    HDF5 and C++ runtime and is never loaded next to the GPU runtime).  Falls
+    back to the oracle restatement when the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
+    import ref_tool
     n = cfg.nx * cfg.ny
     sc = [0.0, cfg.dt, 0.0, initial_carbon, 0.0, 0.0, 0, 0, 0, cfg.initial_mu]
-    kind = "reference"
-    try:
-        if not O.ref_available():
-            raise RuntimeError("no reference build")
-        R = O.Ref(path)
-        R.set_state(phi, c, mu, np.array(sc, dtype=np.float64))
-
-        def step():
-            info, err = R.advance(cfg.end_time)
-            if err:
-                raise RuntimeError(err)
-    except Exception as e:  # noqa: BLE001 - fall back to the oracle restatement
-        log("cpu baseline: reference unavailable (", e, "); timing the oracle restatement")
-        kind = "port"
+    if O.ref_available():
+        state = os.path.join(tmpdir, "baseline_state.npz")
+        np.savez(state, phi=phi, carbon=c, mu=mu, scalars=np.array(sc), stop_time=cfg.end_time)
+        out = os.path.join(tmpdir, "baseline.json")
+        ref_tool.call("baseline", path, state, seconds, out, timeout=int(seconds * 20 + 120))
+        r = json.load(open(out))
+        steps, el, kind = r["steps"], r["seconds"], "reference"
+    else:
+        log("cpu baseline: reference build absent; timing the oracle restatement")
         P = O.params_from_dict(cfg.solver_params())
         S = O.OracleState(phi, c, mu, sc[:9])
-
-        def step():
+        steps, kind = 0, "port"
+        t0 = time.perf_counter()
+        while True:
             rc, _ = O.advance(P, S, cfg.initial_mu, cfg.end_time)
             if rc:
                 raise RuntimeError(f"oracle advance failed {rc}")
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        step()
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
     return dict(value=n * steps / el / 1e6, unit="Mcell-steps/s", cores=1, kind=kind,
                 sample=f"{steps} accepted steps of the same {cfg.nx}x{cfg.ny} workload from the same initial "
                        f"state, serial, {el:.1f} s")
@@ -222,7 +218,7 @@ def main():
     s.close()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(path, cfg, phi, c, mu, initial_carbon, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(path, cfg, phi, c, mu, initial_carbon, args.cpu_seconds, tmp)
     print(json.dumps(out), flush=True)
 
 

@@ -103,3 +103,24 @@ def colony_input(path: str, nx: int, ny: int, pairs: int, strip_rows: int, end_t
           "max_undercooling": repr(float(max(10.0, math.ceil(undercooling))))}
     ov.update(overrides or {})
     return write_input(path, nx, ny, end_time, output_dt if output_dt else end_time * 10, fills, ov)
+
+
+def ensemble(count: int = 64, seed: int = 7, widths=(256, 512, 1024, 2048), heights=(128, 256, 512, 1024),
+             undercooling=(5.0, 40.0), pairs=(2, 16)):
+    """Seeded ensemble of independent colonies: widths and heights drawn from
+    the given sets, undercooling uniform in the given range, 2..16 lamella
+    pairs, colony height a quarter of the domain.  The cementite width
+    follows the lever rule at the run temperature (computed by the caller
+    from the contacts of that temperature; see lever_rule_fraction)."""
+    rng = random.Random(seed)
+    out = []
+    for i in range(count):
+        out.append(dict(index=i, nx=rng.choice(widths), ny=rng.choice(heights),
+                        undercooling=rng.uniform(*undercooling), pairs=rng.randint(*pairs)))
+    return out
+
+
+def lever_rule_fraction(c_mean: float, c_alpha: float, c_theta: float) -> float:
+    """Cementite fraction of ferrite + cementite grown from austenite of mean
+    carbon c_mean (lever rule between the product contacts)."""
+    return (c_mean - c_alpha) / (c_theta - c_alpha)

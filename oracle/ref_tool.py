@@ -1,0 +1,77 @@
+"""Command-line access to the reference solver built from source
+(oracle/_ref/libpearl_ref.so), for processes that also use the GPU: the
+reference library carries its own HDF5/zlib and C++ runtime, so it is only
+ever loaded in a separate process started from here.
+
+TEST INFRASTRUCTURE ONLY (tests/ and the CPU-baseline leg of bench.py).
+
+  python oracle/ref_tool.py read-checkpoint INPUT.in FRAME.h5 OUT.npz
+      load a checkpoint with the reference reader (signature, closure and
+      ledger checks included) and save phi, carbon, mu, scalars
+  python oracle/ref_tool.py baseline INPUT.in STATE.npz SECONDS OUT.json
+      time the reference advance() from the saved state, serially
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def call(*args, timeout=600):
+    """Run a subcommand in a child process; raises on failure."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "ref_tool.py"), *map(str, args)],
+                       capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_tool {args[0]} failed: {r.stderr.strip()[-2000:]}")
+    return r.stdout
+
+
+def read_checkpoint(inp, h5, tmpdir):
+    out = os.path.join(tmpdir, os.path.basename(h5) + ".npz")
+    call("read-checkpoint", inp, h5, out)
+    z = np.load(out)
+    return z["phi"], z["carbon"], z["mu"], z["scalars"]
+
+
+def main(argv):
+    sys.path.insert(0, HERE)
+    import oracle as O
+    cmd = argv[0]
+    if cmd == "read-checkpoint":
+        inp, h5, out = argv[1:4]
+        R = O.Ref(inp)
+        R.read_checkpoint(h5)
+        phi, c, mu, sc = R.get_state()
+        np.savez(out, phi=phi, carbon=c, mu=mu, scalars=sc)
+        return 0
+    if cmd == "baseline":
+        inp, state, seconds, out = argv[1], argv[2], float(argv[3]), argv[4]
+        z = np.load(state)
+        R = O.Ref(inp)
+        R.set_state(z["phi"], z["carbon"], z["mu"], z["scalars"])
+        stop = float(z["stop_time"])
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            info, err = R.advance(stop)
+            if err:
+                raise RuntimeError(err)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
+        with open(out, "w") as f:
+            json.dump({"steps": steps, "seconds": el}, f)
+        return 0
+    raise SystemExit(f"unknown command {cmd}")
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
