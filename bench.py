@@ -103,6 +103,90 @@ def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
                        f"state, serial, {el:.1f} s")
 
 
+def build_ensemble(pkg, count, seed, tmpdir):
+    """Seeded ensemble: widths from {256..2048}, heights from {128..1024},
+    undercooling U[5,40] K, 2..16 lamella pairs, cementite width by the lever
+    rule between the product contacts of the run temperature, colony a
+    quarter of the domain high."""
+    import importlib
+    import numpy as np
+    inputs = importlib.import_module("pearl_amd.inputs")
+    config = importlib.import_module("pearl_amd.config")
+    initial = importlib.import_module("pearl_amd.initial")
+    doms = []
+    for spec in inputs.ensemble(count, seed):
+        nx, ny, uc, pairs = spec["nx"], spec["ny"], spec["undercooling"], spec["pairs"]
+        c_alpha = 0.00088690859310575857 + (-4.6282639461434345e-06) * (-uc)
+        f_theta = inputs.lever_rule_fraction(0.034458832049305582, c_alpha, 0.25)
+        strip = ny // 4
+        fills = inputs.lamella_fills(nx, pairs, strip, theta_fraction=f_theta, spacing_noise=0.05,
+                                     seed=seed * 1000 + spec["index"])
+        path = os.path.join(tmpdir, f"dom{spec['index']}.in")
+        inputs.write_input(path, nx, ny, 1e-6, 1e-6, fills,
+                           overrides={"undercooling": repr(float(uc)), "max_undercooling": "40"})
+        cfg = config.read_config(path)
+        phi = initial.diffuse_colony(nx, ny, cfg.dx, cfg.dy, cfg.eps, initial.plates_from_fills(fills), strip)
+        c = initial.carbon_at_mu(phi, cfg.thermo, cfg.initial_mu)
+        mu = np.full((ny, nx), cfg.initial_mu)
+        doms.append(dict(spec=spec, cfg=cfg, phi=phi, c=c, mu=mu))
+    return doms
+
+
+def run_ensemble(args, pkg):
+    """Batched ensemble in one context versus the same domains one at a time."""
+    import numpy as np
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:  # noqa: BLE001
+        torch, have_torch = None, False
+    tmp = tempfile.mkdtemp()
+    doms = build_ensemble(pkg, args.domains, args.seed, tmp)
+    cells = sum(d["cfg"].nx * d["cfg"].ny for d in doms)
+    log(f"ensemble: {len(doms)} domains, {cells / 1e6:.2f} M cells")
+    b = pkg.Batch([d["cfg"].solver_params() for d in doms], [d["cfg"].initial_mu for d in doms],
+                  cells_per_thread=args.cells_per_thread)
+    for i, d in enumerate(doms):
+        b.set_domain_state(i, d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0])
+    stops = [1e-6] * len(doms)
+    r = b.run_all(stops, [args.warmup] * len(doms))
+    assert r["rc"] == 0, r
+    if have_torch:
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = b.run_all(stops, [args.steps] * len(doms))
+    wall = time.perf_counter() - t0
+    assert r["rc"] == 0, r
+    batched = cells * args.steps / wall / 1e6
+    ctr = b.counters()
+    b.close()
+    log(f"batched: {args.steps} steps per domain in {wall:.3f} s -> {batched:.1f} Mcell-steps/s")
+    # the same domains, one context each, one after the other
+    seq_steps = max(1, args.steps // 4)
+    t_seq = 0.0
+    for d in doms:
+        s = pkg.Solver(d["cfg"].solver_params(), d["cfg"].initial_mu)
+        s.set_state(d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0])
+        s.run(1e-6, args.warmup)
+        t0 = time.perf_counter()
+        s.run(1e-6, seq_steps)
+        t_seq += time.perf_counter() - t0
+        s.close()
+    sequential = cells * seq_steps / t_seq / 1e6
+    log(f"sequential: {seq_steps} steps per domain in {t_seq:.3f} s -> {sequential:.1f} Mcell-steps/s")
+    out = {
+        "metric": "Mcell-steps/s", "value": round(batched, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched ensemble of {len(doms)} uneven domains (seed {args.seed}), FP64, "
+                               f"one context, rounds replayed from a captured graph",
+                   "domains": len(doms), "cells": cells, "cells_per_thread": args.cells_per_thread},
+        "sequential_same_domains": round(sequential, 3), "batched_over_sequential": round(batched / sequential, 2),
+        "attempts": ctr["attempts"], "cg_iterations": ctr["cg_iterations"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +200,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--workload", choices=["production", "ensemble"], default="production")
+    ap.add_argument("--domains", type=int, default=64)
+    ap.add_argument("--cells-per-thread", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -126,6 +213,9 @@ def main():
 
     import numpy as np
     pkg = load_pkg()
+    if args.workload == "ensemble":
+        run_ensemble(args, pkg)
+        return
     try:
         import torch
         have_torch = torch.cuda.is_available()

@@ -256,7 +256,10 @@ void info_of(const DevScal& h, pearl_step_info* info) {
 // Run every domain until it has taken max_steps[d] accepted steps or reached
 // stop_time[d] (the step loop between two output times).  Rounds are
 // replayed in batches between synchronisations; domains that finish idle.
-int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_steps) {
+// With `always_start` a domain starts one step even at or past stop_time,
+// as a direct call of advance does (it then fails with "invalid advance
+// time", :1121).
+int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_steps, bool always_start) {
     int st = read_scalars(c);
     if (st) return st;
     long long most = 0;
@@ -275,7 +278,7 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
         h.cg_peak = 0;
         h.i_dt = h.i_change = h.i_residual = h.i_closure = h.i_energy_change = 0;
         h.i_cg = h.i_retries = 0;
-        const bool work = max_steps[d] > 0 && h.time < stop_time[d];
+        const bool work = max_steps[d] > 0 && (always_start || h.time < stop_time[d]);
         h.phase = work ? PH_START : PH_IDLE;
         if (work) most = std::max(most, max_steps[d]);
     }
@@ -582,12 +585,13 @@ int pearl_get_state(pearl_ctx* c, double* phi, double* carbon, double* mu, pearl
     return pearl_batch_get_state(c, 0, phi, carbon, mu, s);
 }
 
-int pearl_batch_run(pearl_ctx* c, const double* stop_times, const long long* max_steps, long long* taken,
-                    long long* cg_total, int* cg_peak, pearl_step_info* last, int* status) {
+namespace {
+int batch_run(pearl_ctx* c, const double* stop_times, const long long* max_steps, long long* taken,
+              long long* cg_total, int* cg_peak, pearl_step_info* last, int* status, bool always_start) {
     int st = ctx_check(c, 0);
     if (st) return st;
     if (!stop_times || !max_steps) return PEARL_ERR_ARGUMENT;
-    st = run_domains(c, stop_times, max_steps);
+    st = run_domains(c, stop_times, max_steps, always_start);
     if (st) return st;
     int first = PEARL_OK, first_d = -1;
     for (int d = 0; d < c->ndom; ++d) {
@@ -607,13 +611,33 @@ int pearl_batch_run(pearl_ctx* c, const double* stop_times, const long long* max
     return first;
 }
 
+int single_run(pearl_ctx* c, double stop_time, long long max_steps, long long* taken, long long* cg_total,
+               int* cg_peak, pearl_step_info* last, bool always_start) {
+    int status = 0;
+    pearl_step_info info{};
+    long long t = 0, cgt = cg_total ? *cg_total : 0;
+    int cgp = cg_peak ? *cg_peak : 0;
+    const int st = batch_run(c, &stop_time, &max_steps, &t, &cgt, &cgp, &info, &status, always_start);
+    if (taken) *taken = t;
+    if (cg_total) *cg_total = cgt;
+    if (cg_peak) *cg_peak = cgp;
+    if (last) *last = info;
+    return st;
+}
+}  // namespace
+
+int pearl_batch_run(pearl_ctx* c, const double* stop_times, const long long* max_steps, long long* taken,
+                    long long* cg_total, int* cg_peak, pearl_step_info* last, int* status) {
+    return batch_run(c, stop_times, max_steps, taken, cg_total, cg_peak, last, status, false);
+}
+
 int pearl_run(pearl_ctx* c, double stop_time, long long max_steps, long long* taken, long long* cg_total, int* cg_peak,
               pearl_step_info* last) {
     int status = 0;
     pearl_step_info info{};
     long long t = 0, cgt = cg_total ? *cg_total : 0;
     int cgp = cg_peak ? *cg_peak : 0;
-    const int st = pearl_batch_run(c, &stop_time, &max_steps, &t, &cgt, &cgp, &info, &status);
+    const int st = batch_run(c, &stop_time, &max_steps, &t, &cgt, &cgp, &info, &status, false);
     if (taken) *taken = t;
     if (cg_total) *cg_total = cgt;
     if (cg_peak) *cg_peak = cgp;
@@ -625,7 +649,7 @@ int pearl_advance(pearl_ctx* c, double stop_time, pearl_step_info* info) {
     if (!c || !info) return PEARL_ERR_ARGUMENT;
     long long taken = 0;
     std::memset(info, 0, sizeof(*info));
-    return pearl_run(c, stop_time, 1, &taken, nullptr, nullptr, info);
+    return single_run(c, stop_time, 1, &taken, nullptr, nullptr, info, true);
 }
 
 int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double target, double c_gamma, int* shifted,
