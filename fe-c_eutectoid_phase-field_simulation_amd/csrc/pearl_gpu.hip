@@ -138,18 +138,44 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
         }
         ++slot;
     };
-    if (with_begin) hipLaunchKernelGGL(k_begin, gd, dim3(64), 0, c->stream, c->B);
-    ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, c->B, s); });
-    ev(PEARL_KERNEL_PROPOSE, [&](int s) { PEARL_LAUNCH(c, k_propose, g, c->B, s); });
-    ev(PEARL_KERNEL_SETUP, [&](int s) { PEARL_LAUNCH(c, k_setup, g, c->B, s); });
-    ev(PEARL_KERNEL_RESIDUAL0, [&](int s) { PEARL_LAUNCH(c, k_residual0, g, c->B, s); });
+    // finisher kernels: one block per domain
+    const dim3 gf(static_cast<unsigned>(c->ndom));
+    const dim3 tb(TPB);
+    hipStream_t st = c->stream;
+    const Bufs& B = c->B;
+    if (with_begin) hipLaunchKernelGGL(k_begin, gd, dim3(64), 0, st, B);
+    ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) {
+        PEARL_LAUNCH(c, k_state, g, B, s);
+        hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B);
+    });
+    ev(PEARL_KERNEL_PROPOSE, [&](int s) {
+        PEARL_LAUNCH(c, k_propose, g, B, s);
+        hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B);
+    });
+    ev(PEARL_KERNEL_SETUP, [&](int s) { PEARL_LAUNCH(c, k_setup, g, B, s); });
+    ev(PEARL_KERNEL_RESIDUAL0, [&](int s) {
+        PEARL_LAUNCH(c, k_residual0, g, B, s);
+        hipLaunchKernelGGL(k_residual0_fin, gf, tb, 0, st, B);
+    });
     for (int j = 0; j < slots; ++j) {
-        ev(PEARL_KERNEL_CG_APPLY, [&](int s) { PEARL_LAUNCH(c, k_cg_a, g, c->B, s); });
-        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { PEARL_LAUNCH(c, k_cg_b, g, c->B, s); });
+        ev(PEARL_KERNEL_CG_APPLY, [&](int s) {
+            PEARL_LAUNCH(c, k_cg_a, g, B, s);
+            hipLaunchKernelGGL(k_cg_a_fin, gf, tb, 0, st, B);
+        });
+        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) {
+            PEARL_LAUNCH(c, k_cg_b, g, B, s);
+            hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B);
+        });
     }
-    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, c->B, s); });
-    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, c->B, s); });
-    if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, c->stream, c->B);
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) {
+        PEARL_LAUNCH(c, k_finalize, g, B, s);
+        hipLaunchKernelGGL(k_finalize_fin, gf, tb, 0, st, B);
+    });
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) {
+        PEARL_LAUNCH(c, k_phase_only, g, B, s);
+        hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B);
+    });
+    if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, st, B);
 }
 
 // Attribute the timed launches since the last harvest (stream idle) and
@@ -393,6 +419,7 @@ double* dom_mu(pearl_ctx* c, int d, int buf) { return c->B.mu[buf] + c->kp[d].of
 int state_direct(pearl_ctx* c, int d, bool mixture_is_error, double* valid, double* mass, double* energy) {
     const KP& P = c->kp[d];
     PEARL_LAUNCH(c, k_state_direct, dim3(P.nb), c->B, d);
+    hipLaunchKernelGGL(k_state_direct_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, d);
     const int st = read_scalars(c);
     if (st) return st;
     const DevScal& h = c->hS[d];
@@ -662,6 +689,7 @@ int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double ta
     if (!moving_window) return PEARL_OK;
     const KP& P = c->kp[0];
     PEARL_LAUNCH(c, k_front, dim3(P.nb), c->B, 0);
+    hipLaunchKernelGGL(k_front_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0);
     st = read_scalars(c);
     if (st) return st;
     const double fr = c->hS[0].front;
@@ -674,6 +702,7 @@ int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double ta
         const int m = scan_rows * P.nx;
         const int nbs = (m + TPB * c->k - 1) / (TPB * c->k);
         PEARL_LAUNCH(c, k_strip, dim3(nbs), c->B, 0, m, nbs);
+        hipLaunchKernelGGL(k_strip_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0, nbs);
         st = read_scalars(c);
         if (st) return st;
         if (c->hS[0].gmax > 1e-6) return fail(c, PEARL_ERR_UNSAFE_SHIFT, "unsafe shift: removed strip contains gamma");
@@ -707,6 +736,7 @@ int pearl_diagnostics(pearl_ctx* c, pearl_diag* out) {
     if (!out) return PEARL_ERR_ARGUMENT;
     const KP& P = c->kp[0];
     PEARL_LAUNCH(c, k_diagnostics, dim3(P.nb), c->B, 0);
+    hipLaunchKernelGGL(k_diagnostics_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0);
     st = read_scalars(c);
     if (st) return st;
     const DevScal& h = c->hS[0];
@@ -736,6 +766,7 @@ int pearl_propose_phi(pearl_ctx* c, double dt, double* phi_out, double* max_chan
     st = write_scalars(c);
     if (st) return st;
     PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
+    hipLaunchKernelGGL(k_propose_fin, dim3(static_cast<unsigned>(c->ndom)), dim3(TPB), 0, c->stream, c->B);
     st = read_scalars(c);
     if (st) return st;
     const int status = h.status;

@@ -106,10 +106,10 @@ __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b
 // Blocked reduction.  OPS packs 2 bits per quantity: 0 sum, 1 max, 2 min.
 // Stage 1 (caller): each thread folds its cells into v[q] in cell order.
 // Stage 2: 64-lane xor-halving tree, then (w0+w1)+(w2+w3) per block; block
-// partials to part[q*nb+b].  The last block of the domain to arrive (atomic
-// counter) reduces the partials in the same shape: thread t folds partials
-// t, t+256, ... then the same tree.  Returns true in thread 0 of that
-// block, with the final values in v.
+// partials to part[q*nb+b].  Stage 3, in a separate finisher kernel: the
+// partials are reduced in the same shape.  Splitting stages 2 and 3 across
+// a kernel boundary avoids device-scope fences in every block (on a
+// multi-XCD part such a fence writes back the XCD's L2).
 // ---------------------------------------------------------------------------
 template <unsigned OPS>
 __device__ __forceinline__ int op_of(int q) { return (OPS >> (2 * q)) & 3; }
@@ -132,10 +132,10 @@ __device__ __forceinline__ void wave_tree(double (&v)[NQ]) {
     }
 }
 
+// Stage 2 inside a cell kernel: the block's partials (thread 0 writes them).
 template <int NQ, unsigned OPS>
-__device__ bool block_reduce(double (&v)[NQ], double* __restrict__ part, unsigned* counter, int nb, int lb) {
+__device__ __forceinline__ void block_partial(double (&v)[NQ], double* __restrict__ part, int nb, int lb) {
     __shared__ double sh[4][NQ];
-    __shared__ int is_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     wave_tree<NQ, OPS>(v);
     if (lane == 0) {
@@ -149,19 +149,22 @@ __device__ bool block_reduce(double (&v)[NQ], double* __restrict__ part, unsigne
             const int op = op_of<OPS>(q);
             part[q * nb + lb] = comb(op, comb(op, sh[0][q], sh[1][q]), comb(op, sh[2][q], sh[3][q]));
         }
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == static_cast<unsigned>(nb) - 1u);
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+}
+
+// Stage 3 in a finisher kernel (one 256-thread block per domain, launched
+// after the cell kernel, so its partials are visible without fences):
+// thread t folds partials t, t+256, ... then the same tree.  Returns true in
+// thread 0, with the totals in v.
+template <int NQ, unsigned OPS>
+__device__ __forceinline__ bool final_reduce(double (&v)[NQ], const double* __restrict__ part, int nb) {
+    __shared__ double sh[4][NQ];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int op = op_of<OPS>(q);
         double acc = ident(op);
-        for (int b = threadIdx.x; b < nb; b += TPB)
-            acc = comb(op, acc, __hip_atomic_load(&part[q * nb + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        for (int b = threadIdx.x; b < nb; b += TPB) acc = comb(op, acc, part[q * nb + b]);
         v[q] = acc;
     }
     wave_tree<NQ, OPS>(v);
@@ -176,7 +179,6 @@ __device__ bool block_reduce(double (&v)[NQ], double* __restrict__ part, unsigne
         const int op = op_of<OPS>(q);
         v[q] = comb(op, comb(op, sh[0][q], sh[1][q]), comb(op, sh[2][q], sh[3][q]));
     }
-    *counter = 0u;
     return true;
 }
 
@@ -470,23 +472,31 @@ __global__ __launch_bounds__(TPB) void k_state(Bufs B, int slot) {
     PEARL_CELL_LOOP_END
     v[0] = good ? 1.0 : 0.0;
     v[3] = mix_ok ? 0.0 : 1.0;
-    if (block_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-        S->runs[RUN_STATE]++;
-        if (P.check_energy && v[3] > 0) {
-            S->status = ST_MIXTURE;
-            S->phase = PH_IDLE;
-            return;
-        }
-        if (P.check_energy) {
-            S->e0 = v[2] * P.dx * P.dy;
-            S->e0_valid = 1;
-        }
-        if (!P.phase_only) {
-            S->m0 = v[1] * P.dx * P.dy;
-            S->m0_valid = 1;
-        }
-        S->need_state = 0;
+    block_partial<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb, w.lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (!(S->phase == PH_PROPOSE && S->need_state)) return;
+    double v[4];
+    if (!final_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_STATE]++;
+    if (P.check_energy && v[3] > 0) {
+        S->status = ST_MIXTURE;
+        S->phase = PH_IDLE;
+        return;
     }
+    if (P.check_energy) {
+        S->e0 = v[2] * P.dx * P.dy;
+        S->e0_valid = 1;
+    }
+    if (!P.phase_only) {
+        S->m0 = v[1] * P.dx * P.dy;
+        S->m0_valid = 1;
+    }
+    S->need_state = 0;
 }
 
 // P1: propose_phi :961-1002, then the phase-increment test (:1131).
@@ -557,17 +567,26 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a * n + i] = zp[a];
     PEARL_CELL_LOOP_END
-    if (block_reduce<2, (1u | (1u << 2))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-        S->runs[RUN_PROPOSE]++;
-        S->change = v[0];
-        if (v[1] > 0) {
-            S->status = ST_PROJECTION;
-            S->phase = PH_IDLE;
-            return;
-        }
-        S->live = (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change) ? 1 : 0;
-        S->phase = S->live ? PH_SOLVE : PH_DECIDE;
+    block_partial<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, w.lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (S->phase != PH_PROPOSE || S->need_state) return;
+    double v[2];
+    if (!final_reduce<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_PROPOSE]++;
+    S->change = v[0];
+    if (v[1] > 0) {
+        S->status = ST_PROJECTION;
+        S->phase = PH_IDLE;
+        return;
     }
+    // :1131 -- the rest of the attempt runs only if the increment passes
+    S->live = (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change) ? 1 : 0;
+    S->phase = S->live ? PH_SOLVE : PH_DECIDE;
 }
 
 // P2: implicit_diffusion setup :1013-1020.
@@ -640,21 +659,29 @@ __global__ __launch_bounds__(TPB) void k_residual0(Bufs B, int slot) {
     v[1] += rr * zz;
     v[2] += rr * rr;
     PEARL_CELL_LOOP_END
-    if (block_reduce<3, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-        S->runs[RUN_RESIDUAL0]++;
-        if (S->err & ERRF_MIXTURE) {
-            S->status = ST_MIXTURE;
-            S->phase = PH_IDLE;
-            return;
-        }
-        S->rhs2 = v[0];
-        S->rz = v[1];
-        S->rr = v[2];
-        S->tol = P.cg_atol + P.cg_rtol * __builtin_sqrt(v[0]);
-        S->restart = 1;
-        S->pad = 0;
-        cg_head(P, S, 0);
+    block_partial<3, 0u>(v, B.part + P.poff, P.nb, w.lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_residual0_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
+    double v[3];
+    if (!final_reduce<3, 0u>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_RESIDUAL0]++;
+    if (S->err & ERRF_MIXTURE) {
+        S->status = ST_MIXTURE;
+        S->phase = PH_IDLE;
+        return;
     }
+    S->rhs2 = v[0];
+    S->rz = v[1];
+    S->rr = v[2];
+    S->tol = P.cg_atol + P.cg_rtol * __builtin_sqrt(v[0]);
+    S->restart = 1;
+    S->pad = 0;
+    cg_head(P, S, 0);
 }
 
 // PCG kernel A.  In state CG_ITER: the search direction of this iteration
@@ -697,19 +724,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         Ap[i] = a;
         v[0] += pc * a;
         PEARL_CELL_LOOP_END
-        if (block_reduce<1, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-            S->runs[RUN_CG_APPLY]++;
-            const double pAp = v[0], rz = S->rz;
-            S->pAp = pAp;
-            if (!(pAp > 0) || !__builtin_isfinite(pAp) || !__builtin_isfinite(rz)) {
-                S->cg_state = CG_FAIL;
-                S->phase = PH_DECIDE;
-            } else {
-                S->alpha = rz / pAp;
-                S->pidx = pi ^ 1;
-                S->pad = 1;
-            }
-        }
+        block_partial<1, 0u>(v, B.part + P.poff, P.nb, w.lb);
     } else if (state == CG_TRUE) {
         record(B, slot, w.lb, 2);
         const double* __restrict__ rhs = B.rhs + P.off;
@@ -728,18 +743,43 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         v[0] += rr * rr;
         v[1] += rr * zz;
         PEARL_CELL_LOOP_END
-        if (block_reduce<2, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-            S->runs[RUN_TRUE_RESIDUAL]++;
-            S->rr = v[0];
-            const double res = __builtin_sqrt(v[0]);
-            S->cg_res = res;
-            if (res <= S->tol * 1.05) {
-                S->cg_state = CG_OK;
-            } else {
-                S->rz = v[1];
-                S->restart = 1;
-                cg_head(P, S, S->cg_it + 1);
-            }
+        block_partial<2, 0u>(v, B.part + P.poff, P.nb, w.lb);
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (S->phase != PH_SOLVE) return;
+    const int state = S->cg_state;
+    if (state == CG_ITER) {
+        double v[1];
+        if (!final_reduce<1, 0u>(v, B.part + P.poff, P.nb)) return;
+        S->runs[RUN_CG_APPLY]++;
+        const double pAp = v[0], rz = S->rz;
+        S->pAp = pAp;
+        if (!(pAp > 0) || !__builtin_isfinite(pAp) || !__builtin_isfinite(rz)) {
+            S->cg_state = CG_FAIL;
+            S->phase = PH_DECIDE;
+        } else {
+            S->alpha = rz / pAp;
+            S->pidx ^= 1;
+            S->pad = 1;
+        }
+    } else if (state == CG_TRUE) {
+        double v[2];
+        if (!final_reduce<2, 0u>(v, B.part + P.poff, P.nb)) return;
+        S->runs[RUN_TRUE_RESIDUAL]++;
+        S->rr = v[0];
+        const double res = __builtin_sqrt(v[0]);
+        S->cg_res = res;
+        if (res <= S->tol * 1.05) {
+            S->cg_state = CG_OK;
+        } else {
+            S->rz = v[1];
+            S->restart = 1;
+            cg_head(P, S, S->cg_it + 1);
         }
     }
 }
@@ -771,16 +811,24 @@ __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     v[0] += rr * zz;
     v[1] += rr * rr;
     PEARL_CELL_LOOP_END
-    if (block_reduce<2, 0u>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-        S->runs[RUN_CG_UPDATE]++;
-        const double next = v[0];
-        S->beta = next / S->rz;
-        S->rz = next;
-        S->rr = v[1];
-        S->restart = 0;
-        S->pad = 0;
-        cg_head(P, S, S->cg_it + 1);
-    }
+    block_partial<2, 0u>(v, B.part + P.poff, P.nb, w.lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_cg_b_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_ITER && S->pad == 1)) return;
+    double v[2];
+    if (!final_reduce<2, 0u>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_CG_UPDATE]++;
+    const double next = v[0];
+    S->beta = next / S->rz;
+    S->rz = next;
+    S->rr = v[1];
+    S->restart = 0;
+    S->pad = 0;
+    cg_head(P, S, S->cg_it + 1);
 }
 
 // F + A: flux-form carbon update, closure, mu reconstruction (:1082-1093),
@@ -823,14 +871,22 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     v[3] += e;
     PEARL_CELL_LOOP_END
     v[1] = good ? 1.0 : 0.0;
-    if (block_reduce<4, (1u | (2u << 2))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-        S->runs[RUN_FINALIZE]++;
-        S->closure = v[0];
-        S->valid = v[1];
-        S->mass = v[2];
-        S->energy = v[3];
-        S->phase = PH_DECIDE;
-    }
+    block_partial<4, (1u | (2u << 2))>(v, B.part + P.poff, P.nb, w.lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
+    double v[4];
+    if (!final_reduce<4, (1u | (2u << 2))>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_FINALIZE]++;
+    S->closure = v[0];
+    S->valid = v[1];
+    S->mass = v[2];
+    S->energy = v[3];
+    S->phase = PH_DECIDE;
 }
 
 // Phase-only acceptance (:1134-1136): carbon = composition(mu, mixture(phi')),
@@ -866,19 +922,27 @@ __global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
     PEARL_CELL_LOOP_END
     v[1] = good ? 1.0 : 0.0;
     v[4] = mix_ok ? 0.0 : 1.0;
-    if (block_reduce<5, (1u | (2u << 2) | (1u << 8))>(v, B.part + P.poff, &S->counter, P.nb, w.lb)) {
-        S->runs[RUN_FINALIZE]++;
-        if (v[4] > 0) {
-            S->status = ST_MIXTURE;
-            S->phase = PH_IDLE;
-            return;
-        }
-        S->closure = 0;
-        S->valid = v[1];
-        S->mass = v[2];
-        S->energy = v[3];
-        S->phase = PH_DECIDE;
+    block_partial<5, (1u | (2u << 2) | (1u << 8))>(v, B.part + P.poff, P.nb, w.lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_phase_only_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (!(S->phase == PH_SOLVE && P.phase_only)) return;
+    double v[5];
+    if (!final_reduce<5, (1u | (2u << 2) | (1u << 8))>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_FINALIZE]++;
+    if (v[4] > 0) {
+        S->status = ST_MIXTURE;
+        S->phase = PH_IDLE;
+        return;
     }
+    S->closure = 0;
+    S->valid = v[1];
+    S->mass = v[2];
+    S->energy = v[3];
+    S->phase = PH_DECIDE;
 }
 
 // Acceptance tests and the dt controller of advance (:1131-1173).  One
@@ -989,16 +1053,23 @@ __global__ __launch_bounds__(TPB) void k_state_direct(Bufs B, int d) {
     PEARL_CELL_LOOP_END
     v[0] = good ? 1.0 : 0.0;
     v[3] = mix_ok ? 0.0 : 1.0;
-    if (block_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, &S->counter, P.nb, lb)) {
-        S->e_valid = v[0];
-        S->e_mass = v[1];
-        S->e_energy = v[2];
-        S->err = v[3] > 0 ? (S->err | ERRF_MIXTURE_STATE) : (S->err & ~ERRF_MIXTURE_STATE);
-    }
+    block_partial<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_state_direct_fin(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    double v[4];
+    if (!final_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb)) return;
+    S->e_valid = v[0];
+    S->e_mass = v[1];
+    S->e_energy = v[2];
+    S->err = v[3] > 0 ? (S->err | ERRF_MIXTURE_STATE) : (S->err & ~ERRF_MIXTURE_STATE);
 }
 
 // Snapshot reductions (:2267-2282): min/max of carbon and mu, max phase-sum
 // error, carbon sum, energy.
+constexpr unsigned DIAG_OPS = 2u | (1u << 2) | (2u << 4) | (1u << 6) | (1u << 8) | (1u << 14);
 template <int KK>
 __global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
     const KP& P = B.kp[d];
@@ -1026,17 +1097,22 @@ __global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
     v[6] += e;
     PEARL_CELL_LOOP_END
     v[7] = mix_ok ? 0.0 : 1.0;
-    constexpr unsigned OPS = 2u | (1u << 2) | (2u << 4) | (1u << 6) | (1u << 8) | (1u << 14);
-    if (block_reduce<8, OPS>(v, B.part + P.poff, &S->counter, P.nb, lb)) {
-        S->dmin_c = v[0];
-        S->dmax_c = v[1];
-        S->dmin_mu = v[2];
-        S->dmax_mu = v[3];
-        S->dsumerr = v[4];
-        S->dmass = v[5];
-        S->denergy = v[6];
-        S->err = v[7] > 0 ? (S->err | ERRF_MIXTURE_STATE) : (S->err & ~ERRF_MIXTURE_STATE);
-    }
+    block_partial<8, DIAG_OPS>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_diagnostics_fin(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    double v[8];
+    if (!final_reduce<8, DIAG_OPS>(v, B.part + P.poff, P.nb)) return;
+    S->dmin_c = v[0];
+    S->dmax_c = v[1];
+    S->dmin_mu = v[2];
+    S->dmax_mu = v[3];
+    S->dsumerr = v[4];
+    S->dmass = v[5];
+    S->denergy = v[6];
+    S->err = v[7] > 0 ? (S->err | ERRF_MIXTURE_STATE) : (S->err & ~ERRF_MIXTURE_STATE);
 }
 
 // Moving-window front (:2057-2062): max over interior faces of the gamma=1/2
@@ -1055,7 +1131,13 @@ __global__ __launch_bounds__(TPB) void k_front(Bufs B, int d) {
         if (a >= 0 && b < 0) v[0] = dmaxd(v[0], ((y + 1) - .5 + a / (a - b)) * P.dy);
     }
     PEARL_CELL_LOOP_END
-    if (block_reduce<1, 1u>(v, B.part + P.poff, &S->counter, P.nb, lb)) S->front = v[0];
+    block_partial<1, 1u>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_front_fin(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    double v[1];
+    if (final_reduce<1, 1u>(v, B.part + P.poff, P.nb)) B.S[d].front = v[0];
 }
 
 // Max gamma and removed carbon (c*dx)*dy over the first m cells (:2028,
@@ -1074,10 +1156,15 @@ __global__ __launch_bounds__(TPB) void k_strip(Bufs B, int d, int m, int nbs) {
         v[0] = dmaxd(v[0], g[i]);
         v[1] += c[i] * P.dx * P.dy;
     }
-    if (block_reduce<2, 1u>(v, B.part + P.poff, &S->counter, nbs, blockIdx.x)) {
-        S->gmax = v[0];
-        S->removed = v[1];
-    }
+    block_partial<2, 1u>(v, B.part + P.poff, nbs, blockIdx.x);
+}
+
+__global__ __launch_bounds__(TPB) void k_strip_fin(Bufs B, int d, int nbs) {
+    const KP& P = B.kp[d];
+    double v[2];
+    if (!final_reduce<2, 1u>(v, B.part + P.poff, nbs)) return;
+    B.S[d].gmax = v[0];
+    B.S[d].removed = v[1];
 }
 
 // shift_rows row move and gamma fill (:2029-2042) into the other buffer,
