@@ -227,7 +227,7 @@ def main():
     tmp = tempfile.mkdtemp()
     path, cfg, phi, c, mu = build_workload(pkg, nx, ny, args.pairs, strip_rows, args.seed, args.undercooling, tmp)
     log(f"workload {nx}x{ny}: {args.pairs} pairs, colony {strip_rows} rows, {args.undercooling} K")
-    s = pkg.Solver(cfg.solver_params(), cfg.initial_mu)
+    s = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread)
     s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
     initial_carbon = s.total_carbon()
     s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, initial_carbon, 0.0, 0.0, 0, 0, 0])

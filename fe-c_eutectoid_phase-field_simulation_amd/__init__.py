@@ -32,8 +32,8 @@ WHY = ["phase increment", "linear/closure", "composition/phase bounds", "mass co
        "free-energy increase"]
 FP64 = 0
 
-KERNELS = ["propose_phi", "diffusion_setup", "cg_residual0", "cg_apply", "cg_update", "true_residual",
-           "idle", "finalize_accept", "state_reduce", "moving_window"]
+KERNELS = ["propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
+           "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"]
 
 _dp = C.POINTER(C.c_double)
 

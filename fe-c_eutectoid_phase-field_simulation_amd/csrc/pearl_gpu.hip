@@ -27,20 +27,20 @@ const char* kWhy[5] = {"phase increment", "linear/closure", "composition/phase b
                        "free-energy increase"};
 
 const char* kKernelNames[PEARL_KERNEL_COUNT] = {
-    "propose_phi", "diffusion_setup", "cg_residual0", "cg_apply", "cg_update", "true_residual",
-    "idle", "finalize_accept", "state_reduce", "moving_window"};
+    "propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
+    "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"};
 
 // Algorithmic FP64 words per cell of each kernel (each logical array counted
-// once per pass): propose R phi(3),mu W phi'(3); setup R phi'(3),c,mu
-// W chi,icpt,K,rhs,x; residual0 R x,K,chi,rhs W r,diag,z; apply R z,p_old,K,
-// chi W p,Ap; update R x,p,r,Ap,diag W x,r,z; true residual R x,K,chi,rhs,diag
-// W r,z; finalize+accept R phi'(3),x,K,c,icpt,chi W c',mu'; state reduce
+// once per pass): propose R phi(3),mu W phi'(3); prepare R phi'(3),c,mu
+// W chi,icpt,K,rhs,x,diag,r,z; apply R z,p_old,K,chi W p,Ap; update
+// R x,p,r,Ap,diag W x,r,z; true residual R x,K,chi,rhs,diag W r,z;
+// finalize+accept R phi'(3),x,K,c,icpt,chi W c',mu'; state reduce
 // R phi(3),c,mu.
-const double kWords[PEARL_KERNEL_COUNT] = {7, 10, 7, 6, 8, 7, 0, 10, 5, 0};
+const double kWords[PEARL_KERNEL_COUNT] = {7, 13, 6, 8, 7, 0, 10, 5, 0, 0};
 
 // Device run counter of each kernel id (-1: none).
-const int kRunOf[PEARL_KERNEL_COUNT] = {RUN_PROPOSE,       RUN_SETUP, RUN_RESIDUAL0, RUN_CG_APPLY, RUN_CG_UPDATE,
-                                        RUN_TRUE_RESIDUAL, -1,        RUN_FINALIZE,  RUN_STATE,    -1};
+const int kRunOf[PEARL_KERNEL_COUNT] = {RUN_PROPOSE,       RUN_PREPARE, RUN_CG_APPLY, RUN_CG_UPDATE,
+                                        RUN_TRUE_RESIDUAL, -1,          RUN_FINALIZE, RUN_STATE, -1, -1};
 
 }  // namespace
 
@@ -103,6 +103,7 @@ int hip_fail(pearl_ctx* c, hipError_t e, const char* where) {
             case 1: hipLaunchKernelGGL(kname<1>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
             case 2: hipLaunchKernelGGL(kname<2>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
             case 4: hipLaunchKernelGGL(kname<4>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
+            case 16: hipLaunchKernelGGL(kname<16>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
             default: hipLaunchKernelGGL(kname<8>, grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
         }                                                                                                \
     } while (0)
@@ -118,23 +119,28 @@ hipEvent_t get_event(pearl_ctx* c) {
     return e;
 }
 
-// Enqueue one round: begin, state, propose, setup, residual0, slots x
+// Enqueue one round: begin, state, propose, prepare, slots x
 // (cg_a, cg_b), finalize, phase-only, decide.  With `timed`, each cell
-// kernel is bracketed by events and writes its work record into
+// kernel and its finisher are bracketed by events (the cell kernel alone,
+// then the finisher) and the cell kernel writes its work record into
 // kinds[slot].
 void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide) {
     const dim3 g(static_cast<unsigned>(c->total_blocks));
     const dim3 gd((c->ndom + 63) / 64);
     int slot = 0;
-    auto ev = [&](int kid, auto&& fn) {
+    auto ev = [&](int kid, auto&& cell, auto&& fin) {
         if (timed) {
-            hipEvent_t a = get_event(c), b = get_event(c);
+            hipEvent_t a = get_event(c), b = get_event(c), e = get_event(c);
             (void)hipEventRecord(a, c->stream);
-            fn(slot);
+            cell(slot);
             (void)hipEventRecord(b, c->stream);
+            fin();
+            (void)hipEventRecord(e, c->stream);
             c->pending.push_back({kid, slot, a, b});
+            c->pending.push_back({PEARL_KERNEL_FINISH, -1, b, e});
         } else {
-            fn(-1);
+            cell(-1);
+            fin();
         }
         ++slot;
     };
@@ -144,37 +150,22 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     hipStream_t st = c->stream;
     const Bufs& B = c->B;
     if (with_begin) hipLaunchKernelGGL(k_begin, gd, dim3(64), 0, st, B);
-    ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) {
-        PEARL_LAUNCH(c, k_state, g, B, s);
-        hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B);
-    });
-    ev(PEARL_KERNEL_PROPOSE, [&](int s) {
-        PEARL_LAUNCH(c, k_propose, g, B, s);
-        hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B);
-    });
-    ev(PEARL_KERNEL_SETUP, [&](int s) { PEARL_LAUNCH(c, k_setup, g, B, s); });
-    ev(PEARL_KERNEL_RESIDUAL0, [&](int s) {
-        PEARL_LAUNCH(c, k_residual0, g, B, s);
-        hipLaunchKernelGGL(k_residual0_fin, gf, tb, 0, st, B);
-    });
+    ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, B, s); },
+       [&] { hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B); });
+    ev(PEARL_KERNEL_PROPOSE, [&](int s) { PEARL_LAUNCH(c, k_propose, g, B, s); },
+       [&] { hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B); });
+    ev(PEARL_KERNEL_PREPARE, [&](int s) { PEARL_LAUNCH(c, k_prepare, g, B, s); },
+       [&] { hipLaunchKernelGGL(k_prepare_fin, gf, tb, 0, st, B); });
     for (int j = 0; j < slots; ++j) {
-        ev(PEARL_KERNEL_CG_APPLY, [&](int s) {
-            PEARL_LAUNCH(c, k_cg_a, g, B, s);
-            hipLaunchKernelGGL(k_cg_a_fin, gf, tb, 0, st, B);
-        });
-        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) {
-            PEARL_LAUNCH(c, k_cg_b, g, B, s);
-            hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B);
-        });
+        ev(PEARL_KERNEL_CG_APPLY, [&](int s) { PEARL_LAUNCH(c, k_cg_a, g, B, s); },
+           [&] { hipLaunchKernelGGL(k_cg_a_fin, gf, tb, 0, st, B); });
+        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { PEARL_LAUNCH(c, k_cg_b, g, B, s); },
+           [&] { hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B); });
     }
-    ev(PEARL_KERNEL_FINALIZE, [&](int s) {
-        PEARL_LAUNCH(c, k_finalize, g, B, s);
-        hipLaunchKernelGGL(k_finalize_fin, gf, tb, 0, st, B);
-    });
-    ev(PEARL_KERNEL_FINALIZE, [&](int s) {
-        PEARL_LAUNCH(c, k_phase_only, g, B, s);
-        hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B);
-    });
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, B, s); },
+       [&] { hipLaunchKernelGGL(k_finalize_fin, gf, tb, 0, st, B); });
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
+       [&] { hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B); });
     if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, st, B);
 }
 
@@ -186,15 +177,24 @@ void harvest(pearl_ctx* c) {
         (void)hipMemcpy(kinds.data(), c->d_kinds, MAX_SLOTS * sizeof(int), hipMemcpyDeviceToHost);
         for (auto& e : c->pending) {
             int kid = e.kid;
-            const int kind = e.slot < MAX_SLOTS ? kinds[e.slot] : 0;
-            if (kind == 0) kid = PEARL_KERNEL_CG_IDLE;
-            else if (kid == PEARL_KERNEL_CG_APPLY && kind == 2) kid = PEARL_KERNEL_TRUE_RESIDUAL;
+            if (kid != PEARL_KERNEL_FINISH) {
+                const int kind = e.slot < MAX_SLOTS ? kinds[e.slot] : 0;
+                if (kind == 0) kid = PEARL_KERNEL_CG_IDLE;
+                else if (kid == PEARL_KERNEL_CG_APPLY && kind == 2) kid = PEARL_KERNEL_TRUE_RESIDUAL;
+            }
             float t = 0;
             (void)hipEventElapsedTime(&t, e.a, e.b);
             c->ms[kid] += t;
             c->launches[kid]++;
-            c->free_events.push_back(e.a);
-            c->free_events.push_back(e.b);
+        }
+        // every event of the round once (the middle one is shared by a cell
+        // kernel and its finisher)
+        for (auto& e : c->pending) {
+            if (e.kid == PEARL_KERNEL_FINISH) c->free_events.push_back(e.b);
+            else {
+                c->free_events.push_back(e.a);
+                c->free_events.push_back(e.b);
+            }
         }
         c->pending.clear();
     }
@@ -354,11 +354,18 @@ KP make_kp(const pearl_params* params, double mu_reference, int k) {
     P.nx = params->nx;
     P.ny = params->ny;
     P.n = params->nx * params->ny;
-    P.nb = (P.n + TPB * k - 1) / (TPB * k);
+    P.tiles_x = (P.nx + TPB - 1) / TPB;
+    P.nb = P.tiles_x * ((P.ny + k - 1) / k);
     P.dx = params->dx;
     P.dy = params->dy;
     P.dx2 = params->dx * params->dx;
     P.dy2 = params->dy * params->dy;
+    P.rdx = 1.0 / P.dx;
+    P.rdy = 1.0 / P.dy;
+    P.rdx2 = 1.0 / P.dx2;
+    P.rdy2 = 1.0 / P.dy2;
+    P.rVm = 1.0 / params->Vm;
+    P.r3 = 1.0 / 3.0;
     for (int i = 0; i < 3; ++i) {
         P.A[i] = params->A[i];
         P.B[i] = params->B[i];
@@ -367,6 +374,7 @@ KP make_kp(const pearl_params* params, double mu_reference, int k) {
         P.chi[i] = 1.0 / (2.0 * params->A[i]);
         P.qpsi[i] = -0.25 / params->A[i];
         P.twoA[i] = 2.0 * params->A[i];
+        P.rtwoA[i] = 1.0 / P.twoA[i];
         P.tau[i] = params->tau[i];
     }
     P.tau_min = params->tau[0];
@@ -445,8 +453,8 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
         cells += static_cast<long long>(params[d].nx) * params[d].ny;
     }
     int k = cells_per_thread;
-    if (k <= 0) k = cells >= (1LL << 21) ? 4 : 1;
-    if (k != 1 && k != 2 && k != 4 && k != 8) return PEARL_ERR_ARGUMENT;
+    if (k <= 0) k = 4;
+    if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return PEARL_ERR_ARGUMENT;
     if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
     auto* c = new pearl_ctx();
     c->device = device;
@@ -533,8 +541,8 @@ int pearl_destroy(pearl_ctx* c) {
     if (!c) return PEARL_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& e : c->pending) {
-        (void)hipEventDestroy(e.a);
+    for (auto& e : c->pending) {  // a finisher's first event is its cell kernel's second
+        if (e.kid != PEARL_KERNEL_FINISH) (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
@@ -699,9 +707,8 @@ int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double ta
     const int scan_rows = std::max(0, std::min(rows, P.ny));
     double removed = 0;
     if (scan_rows > 0) {
-        const int m = scan_rows * P.nx;
-        const int nbs = (m + TPB * c->k - 1) / (TPB * c->k);
-        PEARL_LAUNCH(c, k_strip, dim3(nbs), c->B, 0, m, nbs);
+        const int nbs = P.tiles_x * ((scan_rows + c->k - 1) / c->k);
+        PEARL_LAUNCH(c, k_strip, dim3(nbs), c->B, 0, scan_rows, nbs);
         hipLaunchKernelGGL(k_strip_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0, nbs);
         st = read_scalars(c);
         if (st) return st;

@@ -11,8 +11,9 @@
 //
 // Control.  advance() (:1119-1174) runs on the device as a small state
 // machine per domain (DevScal::phase).  The host enqueues fixed "rounds" of
-// kernels -- begin, state reductions, propose, setup, residual, PCG slots,
-// finalize, decide -- and every kernel acts only on the domains whose phase
+// kernels -- begin, state reductions, propose, prepare (setup + initial
+// residual), PCG slots, finalize, decide -- and every kernel acts only on the
+// domains whose phase
 // and PCG state call for it.  A round therefore has no host decision in it:
 // it can be replayed from a captured graph, and one launch serves any number
 // of independent domains of different sizes.
@@ -31,11 +32,14 @@ constexpr int NKIND = 16;       // per-domain run counters
 // reference forms them.
 struct KP {
     int nx, ny, n;          // grid; n = nx*ny cells
-    int nb;                 // blocks of this domain in a cell-kernel grid
+    int nb;                 // blocks (tiles) of this domain in a cell-kernel grid
+    int tiles_x;            // tiles per row of tiles: ceil(nx/256)
     int boff;               // first block of this domain in the grid
     long long off;          // first cell of this domain in every plane
     long long poff;         // start of this domain's reduction partials
     double dx, dy, dx2, dy2;  // dx2 = dx*dx (the reference's d.dx*d.dx)
+    // 1/d rounded to nearest for every constant divisor d (div_const)
+    double rdx, rdy, rdx2, rdy2, rVm, rtwoA[3], r3;
     double A[3], B[3], C[3], D[3];
     double chi[3];          // 1.0/(2.0*A)        Parabola::chi   :84
     double qpsi[3];         // -0.25/A             Parabola::psi   :87
@@ -64,8 +68,8 @@ enum { CG_IDLE = 0, CG_ITER = 1, CG_TRUE = 2, CG_OK = 3, CG_FAIL = 4 };
 enum { ST_OK = 0, ST_INVALID_TIME = 1, ST_DT_MIN = 2, ST_RETRY_LIMIT = 3, ST_MIXTURE = 4, ST_PROJECTION = 5 };
 enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4, ERRF_MIXTURE_STATE = 8 };
 // Kernels counted per domain (work actually done).
-enum { RUN_PROPOSE = 0, RUN_SETUP, RUN_RESIDUAL0, RUN_CG_APPLY, RUN_CG_UPDATE, RUN_TRUE_RESIDUAL, RUN_FINALIZE,
-       RUN_STATE, RUN_COUNT };
+enum { RUN_PROPOSE = 0, RUN_PREPARE, RUN_CG_APPLY, RUN_CG_UPDATE, RUN_TRUE_RESIDUAL, RUN_FINALIZE, RUN_STATE,
+       RUN_COUNT };
 
 // Device-resident state of one domain: controller, step scalars, the
 // results of the reductions and the StepInfo of the last accepted step.
@@ -101,6 +105,24 @@ struct DevScal {
 
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }  // std::min
+
+// a/d rounded to nearest, for a positive divisor d fixed at set-up, given
+// y = 1/d rounded to nearest.  q0 = a*y is within 2 ulp of a/d.  One
+// correction q1 = q0 + (a - q0 d) y (each step one rounding) lands within
+// (1/2 + 2^-51) ulp of a/d, so q1 is a faithful rounding.  For a faithful q
+// and a reciprocal with relative error below 2^-53, Markstein's theorem
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990) gives RN(q + (a - d q) y) =
+// RN(a/d): the second correction returns the IEEE quotient.  Valid while
+// a, the quotient and the remainders are normal.  As d > 0 the quotient
+// has the sign of a; copysign restores it for a zero numerator, where the
+// corrections may turn -0 into +0.  One multiply, four FMAs and a bitfield
+// insert instead of the scaled reciprocal iteration of a general division.
+__device__ __forceinline__ double div_const(double a, double d, double y) {
+    const double q0 = a * y;
+    const double q1 = __builtin_fma(__builtin_fma(-q0, d, a), y, q0);
+    const double q2 = __builtin_fma(__builtin_fma(-q1, d, a), y, q1);
+    return __builtin_copysign(q2, a);
+}
 
 // ---------------------------------------------------------------------------
 // Blocked reduction.  OPS packs 2 bits per quantity: 0 sum, 1 max, 2 min.
@@ -155,17 +177,30 @@ __device__ __forceinline__ void block_partial(double (&v)[NQ], double* __restric
 // Stage 3 in a finisher kernel (one 256-thread block per domain, launched
 // after the cell kernel, so its partials are visible without fences):
 // thread t folds partials t, t+256, ... then the same tree.  Returns true in
-// thread 0, with the totals in v.
+// thread 0, with the totals in v.  The partials are fetched FOLD at a time
+// ahead of the fold, so a thread has that many loads in flight instead of
+// one dependent load per partial.
 template <int NQ, unsigned OPS>
 __device__ __forceinline__ bool final_reduce(double (&v)[NQ], const double* __restrict__ part, int nb) {
+    constexpr int FOLD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : 4);
     __shared__ double sh[4][NQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int op = op_of<OPS>(q);
-        double acc = ident(op);
-        for (int b = threadIdx.x; b < nb; b += TPB) acc = comb(op, acc, part[q * nb + b]);
-        v[q] = acc;
+    for (int q = 0; q < NQ; ++q) v[q] = ident(op_of<OPS>(q));
+    for (int b0 = static_cast<int>(threadIdx.x); b0 < nb; b0 += FOLD * TPB) {
+        double t[NQ][FOLD];
+#pragma unroll
+        for (int u = 0; u < FOLD; ++u) {
+            const int b = b0 + u * TPB;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) t[q][u] = b < nb ? part[q * nb + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < FOLD; ++u) {
+            if (b0 + u * TPB >= nb) break;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) v[q] = comb(op_of<OPS>(q), v[q], t[q][u]);
+        }
     }
     wave_tree<NQ, OPS>(v);
     if (lane == 0) {
@@ -229,7 +264,7 @@ __device__ __forceinline__ double local_tau(const KP& P, const double (&p)[3]) {
 }
 
 // project_simplex :491-509; false where the reference throws.
-__device__ __forceinline__ bool project_simplex(const double (&z)[3], double (&out)[3]) {
+__device__ __forceinline__ bool project_simplex(const double (&z)[3], double r3, double (&out)[3]) {
     double s0 = z[0], s1 = z[1], s2 = z[2], t;
     if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
     if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
@@ -241,28 +276,18 @@ __device__ __forceinline__ bool project_simplex(const double (&z)[3], double (&o
     sum += s1;
     { const double cand = (sum - 1.0) / 2.0; if (s1 > cand) { active = 2; threshold = cand; } }
     sum += s2;
-    { const double cand = (sum - 1.0) / 3.0; if (s2 > cand) { active = 3; threshold = cand; } }
+    { const double cand = div_const(sum - 1.0, 3.0, r3); if (s2 > cand) { active = 3; threshold = cand; } }
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = dmaxd(0.0, z[a] - threshold);
     return active != 0;
 }
 
-// Cell coordinates and stencil neighbours: x periodic; at the y edges the
+// Stencil neighbours of cell i in row y: x periodic; at the y edges the
 // neighbour index falls back to the cell itself (the reference's y ghosts
 // are copies of the edge row, :692-693).
 struct Nbr {
-    int x, y, w, e, s, nn;
+    int y, w, e, s, nn;
 };
-__device__ __forceinline__ Nbr neighbours(const KP& P, int i) {
-    Nbr r;
-    r.y = i / P.nx;
-    r.x = i - r.y * P.nx;
-    r.w = r.x == 0 ? i + P.nx - 1 : i - 1;
-    r.e = r.x == P.nx - 1 ? i - P.nx + 1 : i + 1;
-    r.s = r.y == 0 ? i : i - P.nx;
-    r.nn = r.y == P.ny - 1 ? i : i + P.nx;
-    return r;
-}
 
 __device__ __forceinline__ void load_phi(const double* __restrict__ phi, int n, int i, double (&p)[3]) {
     p[0] = phi[i];
@@ -270,33 +295,39 @@ __device__ __forceinline__ void load_phi(const double* __restrict__ phi, int n, 
     p[2] = phi[2 * n + i];
 }
 
-// weights_at :1022-1028
-__device__ __forceinline__ void face_weights(const KP& P, const double* __restrict__ K, int i, const Nbr& b,
+// weights_at :1022-1028, from the diffusivity of the cell and its four
+// neighbours (w, e, s, n).
+__device__ __forceinline__ void face_weights(const KP& P, int y, double k, double kw, double ke, double ks, double kn,
                                              double (&w)[4]) {
-    const double k = K[i];
-    w[0] = .5 * (k + K[b.w]) / P.dx2;
-    w[1] = .5 * (k + K[b.e]) / P.dx2;
-    w[2] = .5 * (k + K[b.s]) / P.dy2;
-    w[3] = .5 * (k + K[b.nn]) / P.dy2;
-    if (b.y == 0) w[2] = 0;
-    if (b.y == P.ny - 1) w[3] = 0;
+    w[0] = div_const(.5 * (k + kw), P.dx2, P.rdx2);
+    w[1] = div_const(.5 * (k + ke), P.dx2, P.rdx2);
+    w[2] = div_const(.5 * (k + ks), P.dy2, P.rdy2);
+    w[3] = div_const(.5 * (k + kn), P.dy2, P.rdy2);
+    if (y == 0) w[2] = 0;
+    if (y == P.ny - 1) w[3] = 0;
 }
 
-// apply :1029-1035
-__device__ __forceinline__ double apply_cell(const KP& P, const double* __restrict__ in, double chi, double dt,
-                                             int i, const Nbr& b, const double (&w)[4]) {
-    const double v = in[i];
-    return chi * v + dt * (w[0] * (v - in[b.w]) + w[1] * (v - in[b.e]) + w[2] * (v - in[b.s]) + w[3] * (v - in[b.nn]));
+// apply :1029-1035 for one cell: chi v + dt sum_f w_f (v - v_f).
+__device__ __forceinline__ double apply_vals(double chi, double dt, const double (&w)[4], double v, double vw, double ve,
+                                             double vs, double vn) {
+    return chi * v + dt * (w[0] * (v - vw) + w[1] * (v - ve) + w[2] * (v - vs) + w[3] * (v - vn));
+}
+
+// K of mixture (:137) alone, in the same operation order.
+__device__ __forceinline__ double mixture_K(const KP& P, const double (&phi)[3]) {
+    double K = 0.0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) K += phi[p] * P.D[p] * P.chi[p];
+    return K;
 }
 
 // Per-cell acceptance quantities of valid_state (:1098-1114) and the local
 // energy of reduced_energy (:799-819): the cell term, then the east and
-// north face terms in the reference's order.
-__device__ __forceinline__ void accept_cell(const KP& P, const double* __restrict__ phi, int i, const Nbr& b,
-                                            double c, double mu, bool& good, double& energy, bool& mix_ok) {
-    const int n = P.n;
-    double p[3];
-    load_phi(phi, n, i, p);
+// (below the top row) north face terms in the reference's order.  p, qe, qn:
+// phases of the cell and of its east and north neighbours.
+__device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], const double (&qe)[3],
+                                            const double (&qn)[3], bool north, double c, double mu, bool& good,
+                                            double& energy, bool& mix_ok) {
     double sum = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -310,7 +341,7 @@ __device__ __forceinline__ void accept_cell(const KP& P, const double* __restric
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (p[a] > .01) {
-                const double cp = (mu - P.B[a]) / P.twoA[a];
+                const double cp = div_const(mu - P.B[a], P.twoA[a], P.rtwoA[a]);
                 good = good && cp >= -1e-10 && cp <= 1 + 1e-10;
             }
     }
@@ -325,30 +356,35 @@ __device__ __forceinline__ void accept_cell(const KP& P, const double* __restric
         const double dc = c - composition(P.mu_ref, m);
         chem += .5 * dc * dc / m.s;
     }
-    double v = chem / P.Vm + P.obs * (p[0] * p[1] + p[0] * p[2] + p[1] * p[2]) + P.tr * p[0] * p[1] * p[2];
-    {
-        double q[3];
-        load_phi(phi, n, b.e, q);
+    double v = div_const(chem, P.Vm, P.rVm) + P.obs * (p[0] * p[1] + p[0] * p[2] + p[1] * p[2]) + P.tr * p[0] * p[1] * p[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int bb = a + 1; bb < 3; ++bb) {
+            const double z = div_const(p[a] * qe[bb] - qe[a] * p[bb], P.dx, P.rdx);
+            v += P.eps_sigma * z * z;
+        }
+    if (north) {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int bb = a + 1; bb < 3; ++bb) {
-                const double z = (p[a] * q[bb] - q[a] * p[bb]) / P.dx;
-                v += P.eps_sigma * z * z;
-            }
-    }
-    if (b.y != P.ny - 1) {
-        double q[3];
-        load_phi(phi, n, b.nn, q);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int bb = a + 1; bb < 3; ++bb) {
-                const double z = (p[a] * q[bb] - q[a] * p[bb]) / P.dy;
+                const double z = div_const(p[a] * qn[bb] - qn[a] * p[bb], P.dy, P.rdy);
                 v += P.eps_sigma * z * z;
             }
     }
     energy = v;
+}
+
+__device__ __forceinline__ void accept_cell(const KP& P, const double* __restrict__ phi, int i, const Nbr& b,
+                                            double c, double mu, bool& good, double& energy, bool& mix_ok) {
+    double p[3], qe[3], qn[3];
+    load_phi(phi, P.n, i, p);
+    load_phi(phi, P.n, b.e, qe);
+    const bool north = b.y != P.ny - 1;
+    if (north) load_phi(phi, P.n, b.nn, qn);
+    else load_phi(phi, P.n, i, qn);
+    accept_vals(P, p, qe, qn, north, c, mu, good, energy, mix_ok);
 }
 
 // ---------------------------------------------------------------------------
@@ -374,21 +410,91 @@ struct Where {
 // Domain and block-in-domain of this block: a round grid covers all
 // domains; a direct launch (only_d >= 0) covers one domain.
 __device__ __forceinline__ Where where(const Bufs& B, int only_d) {
-    if (only_d >= 0) return {only_d, static_cast<int>(blockIdx.x)};
-    const int d = B.blk2dom[blockIdx.x];
-    return {d, static_cast<int>(blockIdx.x) - B.kp[d].boff};
+    const int g = static_cast<int>(blockIdx.x);
+    if (only_d >= 0) return {only_d, g};
+    const int d = B.blk2dom[g];
+    return {d, g - B.kp[d].boff};
 }
 
 __device__ __forceinline__ void record(const Bufs& B, int slot, int lb, int kind) {
     if (slot >= 0 && lb == 0 && threadIdx.x == 0) B.kinds[slot] = kind;
 }
 
-#define PEARL_CELL_LOOP(KK, P, lb)                                            \
-    _Pragma("unroll") for (int j_ = 0; j_ < (KK); ++j_) {                     \
-        const int i = ((lb) * (KK) + j_) * TPB + static_cast<int>(threadIdx.x); \
-        if (i >= (P).n) break;
+// ---------------------------------------------------------------------------
+// Tiles.  Block lb of a domain covers the tile of 256 columns by TY rows at
+// tile column lb % tiles_x, tile row lb / tiles_x.  Thread t owns column
+// 256*(lb % tiles_x) + t and visits its cells of the tile in increasing row:
+// that is the order of each thread's fold in the blocked reduction.  Down a
+// column consecutive cells are one row apart, so a stencil kernel keeps the
+// south, centre and north values of its column in registers and loads one
+// new row per cell; the west and east neighbours are loads from the lines
+// that the neighbouring lanes fetch for their own centres.
+// ---------------------------------------------------------------------------
+struct Col {
+    int x, y0, y1;  // column; rows [y0, y1)
+    int i0;         // first cell
+    int dw, de;     // offsets of the west and east neighbours (x periodic)
+    bool on;        // the column lies inside the domain
+};
 
-#define PEARL_CELL_LOOP_END }
+template <int TY>
+__device__ __forceinline__ Col column(const KP& P, int lb, int rows) {
+    Col c;
+    const int tx = lb % P.tiles_x, ty = lb / P.tiles_x;
+    c.x = tx * TPB + static_cast<int>(threadIdx.x);
+    c.y0 = ty * TY;
+    c.y1 = c.y0 + TY < rows ? c.y0 + TY : rows;
+    c.on = c.x < P.nx;
+    c.i0 = c.y0 * P.nx + c.x;
+    c.dw = c.x == 0 ? P.nx - 1 : -1;
+    c.de = c.x == P.nx - 1 ? 1 - P.nx : 1;
+    return c;
+}
+
+__device__ __forceinline__ Nbr nbr_of(const KP& P, const Col& c, int i, int y) {
+    Nbr r;
+    r.y = y;
+    r.w = i + c.dw;
+    r.e = i + c.de;
+    r.s = y == 0 ? i : i - P.nx;
+    r.nn = y == P.ny - 1 ? i : i + P.nx;
+    return r;
+}
+
+// Visit the column's cells in order: body(i, y).
+template <int TY, class F>
+__device__ __forceinline__ void for_column(const KP& P, const Col& c, F&& body) {
+    if (!c.on) return;
+    int i = c.i0;
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        const int y = c.y0 + r;
+        if (y >= c.y1) break;
+        body(i, y);
+        i += P.nx;
+    }
+}
+
+// March down the column with the south, centre and north values (type T,
+// produced by load(cell)) in registers: body(i, y, south, centre, north).
+// At the bottom and top rows the missing neighbour is the cell itself.
+template <int TY, class T, class L, class F>
+__device__ __forceinline__ void march(const KP& P, const Col& c, L&& load, F&& body) {
+    if (!c.on) return;
+    int i = c.i0;
+    T s = load(c.y0 == 0 ? i : i - P.nx);
+    T m = load(i);
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        const int y = c.y0 + r;
+        if (y >= c.y1) break;
+        const T n = load(y == P.ny - 1 ? i : i + P.nx);
+        body(i, y, s, m, n);
+        s = m;
+        m = n;
+        i += P.nx;
+    }
+}
 
 // Loop head of implicit_diffusion (:1047-1067) for iteration `it` with the
 // current r.r: info.it, info.residual, then the convergence and
@@ -444,8 +550,7 @@ __global__ void k_begin(Bufs B) {
 // valid_state / total_carbon / reduced_energy of one cell of a state
 // (:1096-1116, :784-788, :793-821).
 __device__ __forceinline__ void state_cell(const KP& P, const double* phi, const double* c, const double* mu, int i,
-                                           bool& good, double& mass, double& energy, bool& mix_ok) {
-    const Nbr nb = neighbours(P, i);
+                                           const Nbr& nb, bool& good, double& mass, double& energy, bool& mix_ok) {
     double e;
     const double ci = c[i];
     accept_cell(P, phi, i, nb, ci, mu[i], good, e, mix_ok);
@@ -454,7 +559,7 @@ __device__ __forceinline__ void state_cell(const KP& P, const double* phi, const
 }
 
 // e0 and mass0 of a step (:1122-1123) when they are not cached.
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_state(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
@@ -467,9 +572,10 @@ __global__ __launch_bounds__(TPB) void k_state(Bufs B, int slot) {
     const double* mu = B.mu[cur] + P.off;
     double v[4] = {1.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    state_cell(P, phi, c, mu, i, good, v[1], v[2], mix_ok);
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, w.lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        state_cell(P, phi, c, mu, i, nbr_of(P, col, i, y), good, v[1], v[2], mix_ok);
+    });
     v[0] = good ? 1.0 : 0.0;
     v[3] = mix_ok ? 0.0 : 1.0;
     block_partial<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb, w.lb);
@@ -499,8 +605,70 @@ __global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
     S->need_state = 0;
 }
 
+// propose_phi :961-1002 for one cell: p and the south/north phases from the
+// column registers, west/east loaded.  v[0] = max change, v[1] = projection
+// failure.
+__device__ __forceinline__ void propose_cell(const KP& P, const double* __restrict__ phi, int n, int i, int y,
+                                             const Col& col, const double (&ps)[3], const double (&p)[3],
+                                             const double (&pn)[3], double m, double dt, double* __restrict__ out,
+                                             double (&v)[2]) {
+    double force[3];
+    const double psi0 = psi(P, 2, m);
+    const double dtr[3] = {2.0 * p[1] * p[2], 2.0 * p[0] * p[2], 2.0 * p[0] * p[1]};
+    double dpsi[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) dpsi[b] = psi(P, b, m) - psi0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+        force[a] = P.obs * (p[a1] + p[a2]) + P.tr * p[a1] * p[a2];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double dh = dtr[a] + (b == a ? 6.0 * p[b] * (1.0 - p[b]) : 0.0);
+            force[a] += div_const(dpsi[b] * dh, P.Vm, P.rVm);
+        }
+    }
+    double qw[3], qe[3];
+    load_phi(phi, n, i + col.dw, qw);
+    load_phi(phi, n, i + col.de, qe);
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir) {
+        if ((dir == 2 && y == 0) || (dir == 3 && y == P.ny - 1)) continue;
+        const double(&q)[3] = dir == 0 ? qw : (dir == 1 ? qe : (dir == 2 ? ps : pn));
+        const double dd = dir < 2 ? P.dx2 : P.dy2, rdd = dir < 2 ? P.rdx2 : P.rdy2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 3; ++b) {
+                const double wv = div_const(P.face_coef * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                force[a] += wv * q[b];
+                force[b] -= wv * q[a];
+            }
+    }
+    const double mean = div_const(force[0] + force[1] + force[2], 3.0, P.r3), scale = dt / (local_tau(P, p) * P.eps);
+    double z[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) z[a] = p[a] - scale * (force[a] - mean);
+    if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
+        v[0] = __builtin_inf();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) out[a * n + i] = p[a];
+        return;
+    }
+    double zp[3];
+    if (!project_simplex(z, P.r3, zp)) v[1] = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a * n + i] = zp[a];
+}
+
+struct Phases {
+    double p[3];
+};
+
 // P1: propose_phi :961-1002, then the phase-increment test (:1131).
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
@@ -514,59 +682,17 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     double* __restrict__ out = B.phi[cur ^ 1] + 3 * P.off;
     const double dt = S->dt;
     double v[2] = {0.0, 0.0};
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    const Nbr nb = neighbours(P, i);
-    double p[3], force[3];
-    load_phi(phi, n, i, p);
-    const double m = mu[i];
-    const double psi0 = psi(P, 2, m);
-    const double dtr[3] = {2.0 * p[1] * p[2], 2.0 * p[0] * p[2], 2.0 * p[0] * p[1]};
-    double dpsi[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) dpsi[b] = psi(P, b, m) - psi0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
-        force[a] = P.obs * (p[a1] + p[a2]) + P.tr * p[a1] * p[a2];
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const double dh = dtr[a] + (b == a ? 6.0 * p[b] * (1.0 - p[b]) : 0.0);
-            force[a] += dpsi[b] * dh / P.Vm;
-        }
-    }
-#pragma unroll
-    for (int dir = 0; dir < 4; ++dir) {
-        if ((dir == 2 && nb.y == 0) || (dir == 3 && nb.y == P.ny - 1)) continue;
-        const int j = dir == 0 ? nb.w : (dir == 1 ? nb.e : (dir == 2 ? nb.s : nb.nn));
-        double q[3];
-        load_phi(phi, n, j, q);
-        const double dd = dir < 2 ? P.dx2 : P.dy2;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = a + 1; b < 3; ++b) {
-                const double wv = P.face_coef * (p[a] * q[b] - q[a] * p[b]) / dd;
-                force[a] += wv * q[b];
-                force[b] -= wv * q[a];
-            }
-    }
-    const double mean = (force[0] + force[1] + force[2]) / 3, scale = dt / (local_tau(P, p) * P.eps);
-    double z[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) z[a] = p[a] - scale * (force[a] - mean);
-    if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
-        v[0] = __builtin_inf();
-#pragma unroll
-        for (int a = 0; a < 3; ++a) out[a * n + i] = p[a];
-        continue;
-    }
-    double zp[3];
-    if (!project_simplex(z, zp)) v[1] = 1.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
-#pragma unroll
-    for (int a = 0; a < 3; ++a) out[a * n + i] = zp[a];
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, w.lb, P.ny);
+    march<TY, Phases>(
+        P, col,
+        [&](const int j) {
+            Phases q;
+            load_phi(phi, n, j, q.p);
+            return q;
+        },
+        [&](const int i, const int y, const Phases& s, const Phases& c, const Phases& nn) {
+            propose_cell(P, phi, n, i, y, col, s.p, c.p, nn.p, mu[i], dt, out, v);
+        });
     block_partial<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
 
@@ -589,16 +715,30 @@ __global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
     S->phase = S->live ? PH_SOLVE : PH_DECIDE;
 }
 
-// P2: implicit_diffusion setup :1013-1020.
-template <int KK>
-__global__ __launch_bounds__(TPB) void k_setup(Bufs B, int slot) {
+// Mixture and mu - mu_ref of one cell of the trial state.
+struct Prep {
+    Mix m;
+    double x;
+    bool ok;
+};
+
+// P2: implicit_diffusion setup (:1013-1020) fused with the initial
+// residual, Jacobi diagonal and preconditioned residual (:1036-1045).
+// Every cell computes its own mixture; the diffusivity of the neighbours
+// comes from the column registers (south/north) or from the trial phases of
+// the west/east cell (the same expression, hence the same bits).  Sums
+// rhs.rhs, r.z, r.r and flags an invalid mixture; the finisher forms the
+// tolerance and evaluates the loop head for iteration 0.
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
     record(B, slot, w.lb, 1);
-    if (w.lb == 0 && threadIdx.x == 0) S->runs[RUN_SETUP]++;
+    const int n = P.n;
     const int cur = S->cur;
+    const double dt = S->dt;
     const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
     const double* __restrict__ c_old = B.c[cur] + P.off;
     const double* __restrict__ mu_old = B.mu[cur] + P.off;
@@ -607,70 +747,59 @@ __global__ __launch_bounds__(TPB) void k_setup(Bufs B, int slot) {
     double* __restrict__ K = B.K + P.off;
     double* __restrict__ rhs = B.rhs + P.off;
     double* __restrict__ x = B.x + P.off;
-    bool bad = false;
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    double p[3];
-    load_phi(phi_t, P.n, i, p);
-    Mix m;
-    if (!mixture(P, p, m)) bad = true;
-    const double ic = composition(P.mu_ref, m);
-    chi[i] = m.s;
-    icpt[i] = ic;
-    K[i] = m.K;
-    rhs[i] = c_old[i] - ic;
-    x[i] = mu_old[i] - P.mu_ref;
-    PEARL_CELL_LOOP_END
-    if (bad) atomicOr(&S->err, ERRF_MIXTURE);
-}
-
-// P3: initial residual, Jacobi diagonal, preconditioned residual
-// (:1036-1045).  Sums rhs.rhs, r.z, r.r; the finisher forms the tolerance
-// and evaluates the loop head for iteration 0.
-template <int KK>
-__global__ __launch_bounds__(TPB) void k_residual0(Bufs B, int slot) {
-    const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
-    DevScal* S = B.S + w.d;
-    if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
-    record(B, slot, w.lb, 1);
-    const double dt = S->dt;
-    const double* __restrict__ chi = B.chi + P.off;
-    const double* __restrict__ K = B.K + P.off;
-    const double* __restrict__ rhs = B.rhs + P.off;
-    const double* __restrict__ x = B.x + P.off;
     double* __restrict__ diag = B.diag + P.off;
     double* __restrict__ r = B.r + P.off;
     double* __restrict__ z = B.z + P.off;
-    double v[3] = {0.0, 0.0, 0.0};
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    const Nbr nb = neighbours(P, i);
-    double wf[4];
-    face_weights(P, K, i, nb, wf);
-    const double c = chi[i];
-    const double ax = apply_cell(P, x, c, dt, i, nb, wf);
-    const double d = c + dt * (wf[0] + wf[1] + wf[2] + wf[3]);
-    const double b = rhs[i];
-    const double rr = b - ax;
-    const double zz = rr / d;
-    diag[i] = d;
-    r[i] = rr;
-    z[i] = zz;
-    v[0] += b * b;
-    v[1] += rr * zz;
-    v[2] += rr * rr;
-    PEARL_CELL_LOOP_END
-    block_partial<3, 0u>(v, B.part + P.poff, P.nb, w.lb);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const Col col = column<TY>(P, w.lb, P.ny);
+    march<TY, Prep>(
+        P, col,
+        [&](const int j) {
+            Prep q;
+            double p[3];
+            load_phi(phi_t, n, j, p);
+            q.ok = mixture(P, p, q.m);
+            q.x = mu_old[j] - P.mu_ref;
+            return q;
+        },
+        [&](const int i, const int y, const Prep& s, const Prep& c, const Prep& nn) {
+            if (!c.ok) v[3] = 1.0;
+            const double ic = composition(P.mu_ref, c.m);
+            const double b = c_old[i] - ic;
+            chi[i] = c.m.s;
+            icpt[i] = ic;
+            K[i] = c.m.K;
+            rhs[i] = b;
+            x[i] = c.x;
+            double qw[3], qe[3];
+            load_phi(phi_t, n, i + col.dw, qw);
+            load_phi(phi_t, n, i + col.de, qe);
+            const double xw = mu_old[i + col.dw] - P.mu_ref, xe = mu_old[i + col.de] - P.mu_ref;
+            double wf[4];
+            face_weights(P, y, c.m.K, mixture_K(P, qw), mixture_K(P, qe), s.m.K, nn.m.K, wf);
+            const double ax = apply_vals(c.m.s, dt, wf, c.x, xw, xe, s.x, nn.x);
+            const double d = c.m.s + dt * (wf[0] + wf[1] + wf[2] + wf[3]);
+            const double rr = b - ax;
+            const double zz = rr / d;
+            diag[i] = d;
+            r[i] = rr;
+            z[i] = zz;
+            v[0] += b * b;
+            v[1] += rr * zz;
+            v[2] += rr * rr;
+        });
+    block_partial<4, (1u << 6)>(v, B.part + P.poff, P.nb, w.lb);
 }
 
-__global__ __launch_bounds__(TPB) void k_residual0_fin(Bufs B) {
+__global__ __launch_bounds__(TPB) void k_prepare_fin(Bufs B) {
     const int d = blockIdx.x;
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
-    double v[3];
-    if (!final_reduce<3, 0u>(v, B.part + P.poff, P.nb)) return;
-    S->runs[RUN_RESIDUAL0]++;
-    if (S->err & ERRF_MIXTURE) {
+    double v[4];
+    if (!final_reduce<4, (1u << 6)>(v, B.part + P.poff, P.nb)) return;
+    S->runs[RUN_PREPARE]++;
+    if (v[3] > 0) {
         S->status = ST_MIXTURE;
         S->phase = PH_IDLE;
         return;
@@ -684,13 +813,22 @@ __global__ __launch_bounds__(TPB) void k_residual0_fin(Bufs B) {
     cg_head(P, S, 0);
 }
 
+// Search direction and diffusivity of one cell.
+struct DirK {
+    double p, k;
+};
+// Solution and diffusivity of one cell.
+struct SolK {
+    double x, k;
+};
+
 // PCG kernel A.  In state CG_ITER: the search direction of this iteration
 // p = z + beta p_old (:1079; p = z after a (re)start :1042/:1062), kept in
 // the other p buffer, then Ap = A p (:1068) and p.Ap; the finisher checks
 // for breakdown (:1070) and forms the step length (:1071).  In state
 // CG_TRUE: the true residual r = rhs - A x with z = r/diag (:1053-1062);
 // the finisher accepts (:1056) or restarts from it (:1060-1065).
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
@@ -701,6 +839,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const double* __restrict__ chi = B.chi + P.off;
     const double* __restrict__ K = B.K + P.off;
     double* __restrict__ z = B.z + P.off;
+    const Col col = column<TY>(P, w.lb, P.ny);
     if (state == CG_ITER) {
         record(B, slot, w.lb, 1);
         const int pi = S->pidx;
@@ -709,21 +848,19 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         double* __restrict__ Ap = B.Ap + P.off;
         const bool fresh = S->restart != 0;
         const double beta = S->beta;
+        auto dir = [&](const int j) { return fresh ? z[j] : z[j] + beta * po[j]; };
         double v[1] = {0.0};
-        PEARL_CELL_LOOP(KK, P, w.lb)
-        const Nbr nb = neighbours(P, i);
-        double wf[4];
-        face_weights(P, K, i, nb, wf);
-        const double pc = fresh ? z[i] : z[i] + beta * po[i];
-        const double pw = fresh ? z[nb.w] : z[nb.w] + beta * po[nb.w];
-        const double pe = fresh ? z[nb.e] : z[nb.e] + beta * po[nb.e];
-        const double ps = fresh ? z[nb.s] : z[nb.s] + beta * po[nb.s];
-        const double pq = fresh ? z[nb.nn] : z[nb.nn] + beta * po[nb.nn];
-        const double a = chi[i] * pc + dt * (wf[0] * (pc - pw) + wf[1] * (pc - pe) + wf[2] * (pc - ps) + wf[3] * (pc - pq));
-        pn[i] = pc;
-        Ap[i] = a;
-        v[0] += pc * a;
-        PEARL_CELL_LOOP_END
+        march<TY, DirK>(
+            P, col, [&](const int j) { return DirK{dir(j), K[j]}; },
+            [&](const int i, const int y, const DirK& s, const DirK& c, const DirK& nn) {
+                const int iw = i + col.dw, ie = i + col.de;
+                double wf[4];
+                face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
+                const double a = apply_vals(chi[i], dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
+                pn[i] = c.p;
+                Ap[i] = a;
+                v[0] += c.p * a;
+            });
         block_partial<1, 0u>(v, B.part + P.poff, P.nb, w.lb);
     } else if (state == CG_TRUE) {
         record(B, slot, w.lb, 2);
@@ -732,17 +869,19 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         const double* __restrict__ diag = B.diag + P.off;
         double* __restrict__ r = B.r + P.off;
         double v[2] = {0.0, 0.0};
-        PEARL_CELL_LOOP(KK, P, w.lb)
-        const Nbr nb = neighbours(P, i);
-        double wf[4];
-        face_weights(P, K, i, nb, wf);
-        const double rr = rhs[i] - apply_cell(P, x, chi[i], dt, i, nb, wf);
-        const double zz = rr / diag[i];
-        r[i] = rr;
-        z[i] = zz;
-        v[0] += rr * rr;
-        v[1] += rr * zz;
-        PEARL_CELL_LOOP_END
+        march<TY, SolK>(
+            P, col, [&](const int j) { return SolK{x[j], K[j]}; },
+            [&](const int i, const int y, const SolK& s, const SolK& c, const SolK& nn) {
+                const int iw = i + col.dw, ie = i + col.de;
+                double wf[4];
+                face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
+                const double rr = rhs[i] - apply_vals(chi[i], dt, wf, c.x, x[iw], x[ie], s.x, nn.x);
+                const double zz = rr / diag[i];
+                r[i] = rr;
+                z[i] = zz;
+                v[0] += rr * rr;
+                v[1] += rr * zz;
+            });
         block_partial<2, 0u>(v, B.part + P.poff, P.nb, w.lb);
     }
 }
@@ -787,7 +926,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
 // PCG kernel B, after a step of kernel A: x += alpha p; r -= alpha Ap;
 // z = r/diag; sums r.z and r.r; the finisher forms beta (:1077) and
 // evaluates the loop head of the next iteration.
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
@@ -802,15 +941,16 @@ __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     const double* __restrict__ diag = B.diag + P.off;
     const double alpha = S->alpha;
     double v[2] = {0.0, 0.0};
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    x[i] += alpha * p[i];
-    const double rr = r[i] - alpha * Ap[i];
-    const double zz = rr / diag[i];
-    r[i] = rr;
-    z[i] = zz;
-    v[0] += rr * zz;
-    v[1] += rr * rr;
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, w.lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int) {
+        x[i] += alpha * p[i];
+        const double rr = r[i] - alpha * Ap[i];
+        const double zz = rr / diag[i];
+        r[i] = rr;
+        z[i] = zz;
+        v[0] += rr * zz;
+        v[1] += rr * rr;
+    });
     block_partial<2, 0u>(v, B.part + P.poff, P.nb, w.lb);
 }
 
@@ -831,16 +971,22 @@ __global__ __launch_bounds__(TPB) void k_cg_b_fin(Bufs B) {
     cg_head(P, S, S->cg_it + 1);
 }
 
+// Solution, diffusivity and trial phases of one cell.
+struct Fin {
+    double x, k, p[3];
+};
+
 // F + A: flux-form carbon update, closure, mu reconstruction (:1082-1093),
 // fused with the acceptance sweep on the trial state: valid_state (:1096),
 // total_carbon (:784) and reduced_energy (:793).
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
     record(B, slot, w.lb, 1);
+    const int n = P.n;
     const int cur = S->cur;
     const double dt = S->dt;
     const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
@@ -853,23 +999,36 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     double* __restrict__ mu_new = B.mu[cur ^ 1] + P.off;
     double v[4] = {0.0, 1.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    const Nbr nb = neighbours(P, i);
-    double wf[4];
-    face_weights(P, K, i, nb, wf);
-    const double xi = x[i];
-    const double change = dt * (wf[0] * (x[nb.w] - xi) + wf[1] * (x[nb.e] - xi) + wf[2] * (x[nb.s] - xi) + wf[3] * (x[nb.nn] - xi));
-    const double cn = c_old[i] + change;
-    const double ic = icpt[i], ch = chi[i];
-    v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
-    const double mn = P.mu_ref + (cn - ic) / ch;
-    c_new[i] = cn;
-    mu_new[i] = mn;
-    double e;
-    accept_cell(P, phi_t, i, nb, cn, mn, good, e, mix_ok);
-    v[2] += cn;
-    v[3] += e;
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, w.lb, P.ny);
+    march<TY, Fin>(
+        P, col,
+        [&](const int j) {
+            Fin q;
+            q.x = x[j];
+            q.k = K[j];
+            load_phi(phi_t, n, j, q.p);
+            return q;
+        },
+        [&](const int i, const int y, const Fin& s, const Fin& c, const Fin& nn) {
+            const int iw = i + col.dw, ie = i + col.de;
+            double wf[4];
+            face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
+            const double xi = c.x;
+            const double change =
+                dt * (wf[0] * (x[iw] - xi) + wf[1] * (x[ie] - xi) + wf[2] * (s.x - xi) + wf[3] * (nn.x - xi));
+            const double cn = c_old[i] + change;
+            const double ic = icpt[i], ch = chi[i];
+            v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
+            const double mn = P.mu_ref + (cn - ic) / ch;
+            c_new[i] = cn;
+            mu_new[i] = mn;
+            double qe[3];
+            load_phi(phi_t, n, ie, qe);
+            double e;
+            accept_vals(P, c.p, qe, nn.p, y != P.ny - 1, cn, mn, good, e, mix_ok);
+            v[2] += cn;
+            v[3] += e;
+        });
     v[1] = good ? 1.0 : 0.0;
     block_partial<4, (1u | (2u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
@@ -891,7 +1050,7 @@ __global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
 
 // Phase-only acceptance (:1134-1136): carbon = composition(mu, mixture(phi')),
 // mu unchanged; then valid_state and reduced_energy on the trial state.
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
@@ -905,21 +1064,21 @@ __global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
     double* __restrict__ mu_new = B.mu[cur ^ 1] + P.off;
     double v[5] = {0.0, 1.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK, P, w.lb)
-    const Nbr nb = neighbours(P, i);
-    double p[3];
-    load_phi(phi_t, P.n, i, p);
-    Mix m;
-    if (!mixture(P, p, m)) mix_ok = false;
-    const double mu = mu_old[i];
-    const double cn = composition(mu, m);
-    c_new[i] = cn;
-    mu_new[i] = mu;
-    double e;
-    accept_cell(P, phi_t, i, nb, cn, mu, good, e, mix_ok);
-    v[2] += cn;
-    v[3] += e;
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, w.lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        double p[3];
+        load_phi(phi_t, P.n, i, p);
+        Mix m;
+        if (!mixture(P, p, m)) mix_ok = false;
+        const double mu = mu_old[i];
+        const double cn = composition(mu, m);
+        c_new[i] = cn;
+        mu_new[i] = mu;
+        double e;
+        accept_cell(P, phi_t, i, nbr_of(P, col, i, y), cn, mu, good, e, mix_ok);
+        v[2] += cn;
+        v[3] += e;
+    });
     v[1] = good ? 1.0 : 0.0;
     v[4] = mix_ok ? 0.0 : 1.0;
     block_partial<5, (1u | (2u << 2) | (1u << 8))>(v, B.part + P.poff, P.nb, w.lb);
@@ -1037,7 +1196,7 @@ __global__ void k_clear_kinds(int* kinds, int n) {
 // ---------------------------------------------------------------------------
 
 // valid_state, total_carbon and reduced_energy of domain d's current state.
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_state_direct(Bufs B, int d) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
@@ -1048,9 +1207,10 @@ __global__ __launch_bounds__(TPB) void k_state_direct(Bufs B, int d) {
     const double* mu = B.mu[cur] + P.off;
     double v[4] = {1.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK, P, lb)
-    state_cell(P, phi, c, mu, i, good, v[1], v[2], mix_ok);
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        state_cell(P, phi, c, mu, i, nbr_of(P, col, i, y), good, v[1], v[2], mix_ok);
+    });
     v[0] = good ? 1.0 : 0.0;
     v[3] = mix_ok ? 0.0 : 1.0;
     block_partial<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb, lb);
@@ -1070,7 +1230,7 @@ __global__ __launch_bounds__(TPB) void k_state_direct_fin(Bufs B, int d) {
 // Snapshot reductions (:2267-2282): min/max of carbon and mu, max phase-sum
 // error, carbon sum, energy.
 constexpr unsigned DIAG_OPS = 2u | (1u << 2) | (2u << 4) | (1u << 6) | (1u << 8) | (1u << 14);
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
@@ -1081,21 +1241,21 @@ __global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
     const double* mu = B.mu[cur] + P.off;
     double v[8] = {1e300, -1e300, 1e300, -1e300, 0.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
-    PEARL_CELL_LOOP(KK, P, lb)
-    const Nbr nb = neighbours(P, i);
-    const double ci = c[i], mi = mu[i];
-    v[0] = dmind(v[0], ci);
-    v[1] = dmaxd(v[1], ci);
-    v[2] = dmind(v[2], mi);
-    v[3] = dmaxd(v[3], mi);
-    double p[3];
-    load_phi(phi, P.n, i, p);
-    v[4] = dmaxd(v[4], __builtin_fabs(p[0] + p[1] + p[2] - 1));
-    double e;
-    accept_cell(P, phi, i, nb, ci, mi, good, e, mix_ok);
-    v[5] += ci;
-    v[6] += e;
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        const double ci = c[i], mi = mu[i];
+        v[0] = dmind(v[0], ci);
+        v[1] = dmaxd(v[1], ci);
+        v[2] = dmind(v[2], mi);
+        v[3] = dmaxd(v[3], mi);
+        double p[3];
+        load_phi(phi, P.n, i, p);
+        v[4] = dmaxd(v[4], __builtin_fabs(p[0] + p[1] + p[2] - 1));
+        double e;
+        accept_cell(P, phi, i, nbr_of(P, col, i, y), ci, mi, good, e, mix_ok);
+        v[5] += ci;
+        v[6] += e;
+    });
     v[7] = mix_ok ? 0.0 : 1.0;
     block_partial<8, DIAG_OPS>(v, B.part + P.poff, P.nb, lb);
 }
@@ -1117,20 +1277,20 @@ __global__ __launch_bounds__(TPB) void k_diagnostics_fin(Bufs B, int d) {
 
 // Moving-window front (:2057-2062): max over interior faces of the gamma=1/2
 // crossing height.
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_front(Bufs B, int d) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const double* g = B.phi[S->cur] + 3 * P.off + 2 * static_cast<long long>(P.n);
     double v[1] = {-1.0};
-    PEARL_CELL_LOOP(KK, P, lb)
-    const int y = i / P.nx;
-    if (y < P.ny - 1) {
-        const double a = 1 - 2 * g[i], b = 1 - 2 * g[i + P.nx];
-        if (a >= 0 && b < 0) v[0] = dmaxd(v[0], ((y + 1) - .5 + a / (a - b)) * P.dy);
-    }
-    PEARL_CELL_LOOP_END
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        if (y < P.ny - 1) {
+            const double a = 1 - 2 * g[i], b = 1 - 2 * g[i + P.nx];
+            if (a >= 0 && b < 0) v[0] = dmaxd(v[0], ((y + 1) - .5 + a / (a - b)) * P.dy);
+        }
+    });
     block_partial<1, 1u>(v, B.part + P.poff, P.nb, lb);
 }
 
@@ -1140,23 +1300,24 @@ __global__ __launch_bounds__(TPB) void k_front_fin(Bufs B, int d) {
     if (final_reduce<1, 1u>(v, B.part + P.poff, P.nb)) B.S[d].front = v[0];
 }
 
-// Max gamma and removed carbon (c*dx)*dy over the first m cells (:2028,
-// :2067).  Launched with nbs = ceil(m/(256*KK)) blocks.
-template <int KK>
-__global__ __launch_bounds__(TPB) void k_strip(Bufs B, int d, int m, int nbs) {
+// Max gamma and removed carbon (c*dx)*dy over the first `rows` rows (:2028,
+// :2067), in the tile order of those rows.  Launched with
+// nbs = tiles_x * ceil(rows/TY) blocks.
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_strip(Bufs B, int d, int rows, int nbs) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
     const int cur = S->cur;
     const double* g = B.phi[cur] + 3 * P.off + 2 * static_cast<long long>(P.n);
     const double* c = B.c[cur] + P.off;
     double v[2] = {0.0, 0.0};
-    _Pragma("unroll") for (int j_ = 0; j_ < KK; ++j_) {
-        const int i = (static_cast<int>(blockIdx.x) * KK + j_) * TPB + static_cast<int>(threadIdx.x);
-        if (i >= m) break;
+    const Col col = column<TY>(P, lb, rows);
+    for_column<TY>(P, col, [&](const int i, const int) {
         v[0] = dmaxd(v[0], g[i]);
         v[1] += c[i] * P.dx * P.dy;
-    }
-    block_partial<2, 1u>(v, B.part + P.poff, nbs, blockIdx.x);
+    });
+    block_partial<2, 1u>(v, B.part + P.poff, nbs, lb);
 }
 
 __global__ __launch_bounds__(TPB) void k_strip_fin(Bufs B, int d, int nbs) {
@@ -1169,7 +1330,7 @@ __global__ __launch_bounds__(TPB) void k_strip_fin(Bufs B, int d, int nbs) {
 
 // shift_rows row move and gamma fill (:2029-2042) into the other buffer,
 // then close_mu (:789-792) on the new fields.
-template <int KK>
+template <int TY>
 __global__ __launch_bounds__(TPB) void k_shift(Bufs B, int d, int rows, double c_gamma) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
@@ -1182,27 +1343,27 @@ __global__ __launch_bounds__(TPB) void k_shift(Bufs B, int d, int rows, double c
     double* __restrict__ c_out = B.c[cur ^ 1] + P.off;
     double* __restrict__ mu_out = B.mu[cur ^ 1] + P.off;
     bool bad_mix = false, bad_c = false;
-    PEARL_CELL_LOOP(KK, P, lb)
-    const int y = i / P.nx;
-    double p[3], c;
-    if (y + rows < P.ny) {
-        const int j = i + rows * P.nx;
-        load_phi(phi_in, n, j, p);
-        c = c_in[j];
-    } else {
-        p[0] = 0;
-        p[1] = 0;
-        p[2] = 1;
-        c = c_gamma;
-    }
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        double p[3], c;
+        if (y + rows < P.ny) {
+            const int j = i + rows * P.nx;
+            load_phi(phi_in, n, j, p);
+            c = c_in[j];
+        } else {
+            p[0] = 0;
+            p[1] = 0;
+            p[2] = 1;
+            c = c_gamma;
+        }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) phi_out[a * n + i] = p[a];
-    c_out[i] = c;
-    Mix m;
-    if (!mixture(P, p, m)) bad_mix = true;
-    if (!__builtin_isfinite(c)) bad_c = true;
-    mu_out[i] = (c - m.c0) / m.s;
-    PEARL_CELL_LOOP_END
+        for (int a = 0; a < 3; ++a) phi_out[a * n + i] = p[a];
+        c_out[i] = c;
+        Mix m;
+        if (!mixture(P, p, m)) bad_mix = true;
+        if (!__builtin_isfinite(c)) bad_c = true;
+        mu_out[i] = (c - m.c0) / m.s;
+    });
     if (bad_mix) atomicOr(&S->err, ERRF_MIXTURE);
     if (bad_c) atomicOr(&S->err, ERRF_NONFINITE_CARBON);
 }

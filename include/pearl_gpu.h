@@ -93,8 +93,10 @@ typedef struct {
 typedef struct pearl_ctx pearl_ctx;
 
 /* Create a context on HIP device `device` for one domain.  mu_reference is
- * initialize()'s return value (:1761, :2231).  cells_per_thread selects the
- * blocked reduction layout (0 = automatic).  Replaces the construction of
+ * initialize()'s return value (:1761, :2231).  cells_per_thread (1, 2, 4, 8
+ * or 16; 0 = automatic) is the number of rows of a tile: each thread of a
+ * cell kernel owns that many cells of one column.  It fixes the blocked
+ * reduction order (pearl_get_layout).  Replaces the construction of
  * Domain/State (:2209, :2226). */
 int pearl_create(const pearl_params* params, double mu_reference, int precision, int device,
                  int cells_per_thread, pearl_ctx** out);
@@ -141,10 +143,12 @@ int pearl_total_carbon(pearl_ctx* ctx, double* mass);
 int pearl_valid_state(pearl_ctx* ctx, int* valid);
 
 /* Blocked-reduction layout in use: threads per block (256) and cells per
- * thread.  A sum over n cells is: block b owns cells [b*256*k, (b+1)*256*k);
- * thread t adds cells b*256*k + j*256 + t for j = 0..k-1 in order; a 64-lane
- * halving tree per wavefront; (w0+w1)+(w2+w3) per block; the block partials
- * are reduced by one 256-thread block in the same way. */
+ * thread k.  A sum over an nx-by-ny domain is: tiles of 256 columns by k
+ * rows, numbered row of tiles by row of tiles (b = ty*ceil(nx/256) + tx);
+ * thread t of tile b owns column 256*tx + t and adds its cells of the tile
+ * in increasing row index; a 64-lane halving tree per wavefront;
+ * (w0+w1)+(w2+w3) per block; the tile partials are reduced by one
+ * 256-thread block in the same way (thread t adds partials t, t+256, ...). */
 int pearl_get_layout(const pearl_ctx* ctx, int* threads_per_block, int* cells_per_thread);
 
 /* Per-kernel timing with HIP events on the context's stream.  Kernel ids:
@@ -155,15 +159,15 @@ int pearl_get_layout(const pearl_ctx* ctx, int* threads_per_block, int* cells_pe
  * PCG solve that had already finished) are counted under
  * PEARL_KERNEL_CG_IDLE with zero bytes. */
 #define PEARL_KERNEL_PROPOSE 0        /* propose_phi */
-#define PEARL_KERNEL_SETUP 1          /* diffusion setup */
-#define PEARL_KERNEL_RESIDUAL0 2      /* initial residual, diagonal, tolerance */
-#define PEARL_KERNEL_CG_APPLY 3       /* direction update + A p + p.Ap */
-#define PEARL_KERNEL_CG_UPDATE 4      /* x, r, z updates + r.z, r.r */
-#define PEARL_KERNEL_TRUE_RESIDUAL 5  /* true residual at convergence (+ restart) */
-#define PEARL_KERNEL_CG_IDLE 6        /* speculative PCG launches that found the solve finished */
-#define PEARL_KERNEL_FINALIZE 7       /* flux update, closure, mu + acceptance reductions */
-#define PEARL_KERNEL_STATE_REDUCE 8   /* valid/mass/energy of the current state, diagnostics */
-#define PEARL_KERNEL_WINDOW 9         /* moving-window front, strip, shift */
+#define PEARL_KERNEL_PREPARE 1        /* diffusion setup + initial residual, diagonal, tolerance */
+#define PEARL_KERNEL_CG_APPLY 2       /* direction update + A p + p.Ap */
+#define PEARL_KERNEL_CG_UPDATE 3      /* x, r, z updates + r.z, r.r */
+#define PEARL_KERNEL_TRUE_RESIDUAL 4  /* true residual at convergence (+ restart) */
+#define PEARL_KERNEL_CG_IDLE 5        /* speculative PCG launches that found the solve finished */
+#define PEARL_KERNEL_FINALIZE 6       /* flux update, closure, mu + acceptance reductions */
+#define PEARL_KERNEL_STATE_REDUCE 7   /* valid/mass/energy of the current state, diagnostics */
+#define PEARL_KERNEL_WINDOW 8         /* moving-window front, strip, shift */
+#define PEARL_KERNEL_FINISH 9         /* one-block finishers of the blocked reductions */
 #define PEARL_KERNEL_COUNT 10
 int pearl_profile(pearl_ctx* ctx, int enable);
 int pearl_kernel_stats(pearl_ctx* ctx, int kernel, long long* launches, double* total_ms, double* bytes);

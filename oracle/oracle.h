@@ -12,12 +12,13 @@
  *   ORC_RED_SEQ      row-major sequential sums, as the serial reference
  *                    (array_dot pearl_serial.cpp:738-742, total_carbon :784-788,
  *                    reduced_energy :798-820).  Bitwise equal to pearl_serial.
- *   ORC_RED_BLOCKED  the canonical blocked order of the HIP kernels: blocks of
- *                    256 threads own 256*k consecutive row-major cells; thread
- *                    t adds cells base+j*256+t (j=0..k-1) in order; 64-lane
- *                    halving tree per wavefront; (w0+w1)+(w2+w3) per block;
- *                    block partials reduced by one 256-thread block the same
- *                    way (thread t adds partials t, t+256, ...).
+ *   ORC_RED_BLOCKED  the canonical blocked order of the HIP kernels: tiles of
+ *                    256 columns by k rows (red_k), numbered row of tiles by
+ *                    row of tiles; thread t of a tile owns one column and adds
+ *                    its k cells in increasing row; 64-lane halving tree per
+ *                    wavefront; (w0+w1)+(w2+w3) per block; tile partials
+ *                    reduced by one 256-thread block the same way (thread t
+ *                    adds partials t, t+256, ...).
  * Max/min/AND reductions are order-independent and identical in both.
  *
  * Parity pins (see tests/): ORC_RED_SEQ results are compared bitwise with the
@@ -75,7 +76,7 @@ typedef struct {
     double energy_rtol, energy_atol;
     int cg_max, max_retries, check_energy, phase_only;
     int red_mode;                            /* ORC_RED_SEQ or ORC_RED_BLOCKED */
-    int red_k;                               /* cells per thread (blocked order) */
+    int red_k;                               /* rows per tile = cells per thread (blocked order) */
 } orc_params;
 
 typedef struct {

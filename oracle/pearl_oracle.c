@@ -31,19 +31,28 @@ static double block_tree(double* a) {
     return (w[0] + w[1]) + (w[2] + w[3]);
 }
 
-static double blocked_sum(const double* v, size_t n, int k) {
-    if (k < 1) k = 1;
-    const size_t cpb = (size_t)RED_THREADS * (size_t)k;
-    const size_t nb = n == 0 ? 1 : (n + cpb - 1) / cpb;
+/* Sum of a row-major [rows][nx] array in the canonical blocked order: tiles
+ * of 256 columns by ty rows, numbered row of tiles by row of tiles
+ * (b = tile_row*tiles_x + tile_col); thread t of tile b owns column
+ * 256*tile_col + t and adds its cells of the tile in increasing row; then the
+ * block tree; then the tile partials reduced by one 256-thread block the
+ * same way (thread t adds partials t, t+256, ...). */
+static double blocked_sum(const double* v, size_t n, int nx, int ty) {
+    if (ty < 1) ty = 1;
+    if (nx < 1 || n % (size_t)nx != 0) nx = (int)(n == 0 ? 1 : n);
+    const size_t rows = n / (size_t)nx;
+    const size_t tiles_x = ((size_t)nx + RED_THREADS - 1) / RED_THREADS;
+    const size_t tiles_y = rows == 0 ? 1 : (rows + (size_t)ty - 1) / (size_t)ty;
+    const size_t nb = tiles_x * tiles_y;
     double* part = (double*)malloc(nb * sizeof(double));
     double acc[RED_THREADS];
     for (size_t b = 0; b < nb; ++b) {
+        const size_t tx = b % tiles_x, y0 = (b / tiles_x) * (size_t)ty;
         for (int t = 0; t < RED_THREADS; ++t) {
+            const size_t x = tx * RED_THREADS + (size_t)t;
             double s = 0.0;
-            for (int j = 0; j < k; ++j) {
-                size_t i = b * cpb + (size_t)j * RED_THREADS + (size_t)t;
-                if (i < n) s += v[i];
-            }
+            if (x < (size_t)nx)
+                for (size_t y = y0; y < y0 + (size_t)ty && y < rows; ++y) s += v[y * (size_t)nx + x];
             acc[t] = s;
         }
         part[b] = block_tree(acc);
@@ -58,7 +67,7 @@ static double blocked_sum(const double* v, size_t n, int k) {
 }
 
 double orc_sum(const orc_params* P, const double* v, size_t n) {
-    if (P->red_mode == ORC_RED_BLOCKED) return blocked_sum(v, n, P->red_k);
+    if (P->red_mode == ORC_RED_BLOCKED) return blocked_sum(v, n, P->nx, P->red_k);
     double s = 0;
     for (size_t i = 0; i < n; ++i) s += v[i];
     return s;
@@ -68,7 +77,7 @@ double orc_sum(const orc_params* P, const double* v, size_t n) {
 static double dot(const orc_params* P, const double* a, const double* b, size_t n, double* tmp) {
     if (P->red_mode == ORC_RED_BLOCKED) {
         for (size_t i = 0; i < n; ++i) tmp[i] = a[i] * b[i];
-        return blocked_sum(tmp, n, P->red_k);
+        return blocked_sum(tmp, n, P->nx, P->red_k);
     }
     double s = 0;
     for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
@@ -449,7 +458,7 @@ double orc_reduced_energy(const orc_params* P, const double* phi, const double* 
             if (!seq) cellv[i] = v;
         }
     if (!seq) {
-        total = blocked_sum(cellv, n, P->red_k);
+        total = blocked_sum(cellv, n, P->nx, P->red_k);
         free(cellv);
     }
     return total * P->dx * P->dy;
@@ -594,7 +603,7 @@ int orc_maybe_shift(const orc_params* P, orc_state* s, int moving_window, double
         const size_t m = (size_t)rows * nx;
         double* tmp = (double*)malloc(m * sizeof(double));
         for (size_t i = 0; i < m; ++i) tmp[i] = s->carbon[i] * P->dx * P->dy;
-        removed = blocked_sum(tmp, m, P->red_k);
+        removed = blocked_sum(tmp, m, P->nx, P->red_k);
         free(tmp);
     } else {
         removed = 0;
@@ -661,7 +670,7 @@ static double interface_energy(const orc_params* P, const double* phi) {
             if (!seq) cellv[i] = v;
         }
     if (!seq) {
-        total = blocked_sum(cellv, n, P->red_k);
+        total = blocked_sum(cellv, n, P->nx, P->red_k);
         free(cellv);
     }
     return total * P->dx * P->dy;

@@ -1,10 +1,11 @@
 // Test-only probe of FP64 rounding on the device: square root and division
-// as compiled for the solver (same flags), and the constant-divisor quotient
-// q = RN(a*y); r = fma(-q, d, a); q' = fma(r, y, q) with y = RN(1/d)
-// (Markstein, "Computation of elementary functions on the IBM RISC
-// System/6000 processor", IBM J. Res. Dev. 34(1), 1990).  The host compares
-// the results with correctly rounded host arithmetic.
+// as compiled for the solver (same flags), and the solver's own
+// constant-divisor quotient pearl_gpu::div_const (two Markstein corrections
+// of a*RN(1/d)).  The host compares the results with correctly rounded host
+// arithmetic.
 #include <hip/hip_runtime.h>
+
+#include "pearl_kernels.hpp"
 
 namespace {
 __global__ void k_sqrt(const double* __restrict__ x, double* __restrict__ y, int n) {
@@ -17,12 +18,7 @@ __global__ void k_div(const double* __restrict__ a, const double* __restrict__ b
 }
 __global__ void k_div_const(const double* __restrict__ a, double d, double y, double* __restrict__ q, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const double x = a[i];
-        const double q0 = x * y;
-        const double r = __builtin_fma(-q0, d, x);
-        q[i] = __builtin_fma(r, y, q0);
-    }
+    if (i < n) q[i] = pearl_gpu::div_const(a[i], d, y);
 }
 
 template <typename F>

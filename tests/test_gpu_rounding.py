@@ -1,7 +1,10 @@
 """Device FP64 rounding as compiled for the solver: sqrt and division must
 be correctly rounded (bitwise equal to the host's IEEE results), and the
-constant-divisor quotient used for loop-invariant divisors must equal the
-IEEE quotient for the divisors and numerator ranges of the solver."""
+solver's constant-divisor quotient (div_const) must equal the IEEE quotient:
+random numerators over a wide exponent range, numerators whose quotients
+lie next to rounding midpoints, signed zeros, for the solver's divisors
+(dx, dy, dx^2, dy^2, Vm, 2A, 3) and for divisors with extreme
+significands."""
 import ctypes as C
 import os
 
@@ -54,13 +57,22 @@ def test_division_correctly_rounded():
     assert np.array_equal(q, a / b)
 
 
-@pytest.mark.parametrize("d", [2.79e-9 * 2.79e-9, 1e-6, 3.0, 7.7841e-18, 1.2345678901234567])
+@pytest.mark.parametrize("d", [2.79e-9, 2.79e-9 * 2.79e-9, 1e-9, 1e-9 * 1e-9, 7.1e-6, 3.0, 2.0 * 13021.0,
+                               2.0 * 1.5e5, 7.7841e-18, 1.2345678901234567, float.fromhex("0x1.fffffffffffffp0"),
+                               float.fromhex("0x1.0000000000001p0"), float.fromhex("0x1.fffffffffffffp-40")])
 def test_constant_divisor_quotient(d):
     rng = np.random.default_rng(9)
-    a = np.concatenate([random_doubles(rng, 1 << 22, -250, 250) * rng.choice([-1, 1], 1 << 22),
-                        rng.normal(size=1 << 20), rng.normal(size=1 << 20) * 1e-20])
+    n = 1 << 21
+    a = np.concatenate([random_doubles(rng, n, -250, 250) * rng.choice([-1, 1], n),
+                        rng.normal(size=n), rng.normal(size=n) * 1e-20,
+                        # quotients next to rounding midpoints: a = (m + 1/2 ulp) * d rounded
+                        (random_doubles(rng, n, -40, 40) * d),
+                        np.nextafter(random_doubles(rng, n, -40, 40) * d, np.inf),
+                        np.array([0.0, -0.0, 1.0, -1.0, d, -d])])
     a = np.ascontiguousarray(a)
     q = np.empty_like(a)
     assert lib().probe_div_const(p(a), d, 1.0 / d, p(q), a.size) == 0
-    bad = np.flatnonzero(q != a / d)
+    ref = a / d
+    bad = np.flatnonzero(q != ref)
     assert bad.size == 0, (bad.size, a[bad[:5]])
+    assert np.array_equal(np.signbit(q), np.signbit(ref))

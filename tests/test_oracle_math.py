@@ -53,32 +53,36 @@ def test_project_simplex_matches_bruteforce(O):
 
 
 def test_blocked_sum_tree(O):
-    """The blocked order on small arrays equals an explicit restatement of
-    the tree, and stays within n*eps of the exact sum."""
+    """The blocked order (tiles of 256 columns by k rows, one column per
+    thread) on small arrays equals an explicit restatement of the tree, and
+    stays within n*eps of the exact sum."""
     rng = np.random.default_rng(2)
-    for n, k in [(1, 1), (255, 1), (256, 1), (1000, 2), (5000, 4), (70000, 1), (70000, 8)]:
+    for nx, ny, k in [(1, 1, 1), (255, 1, 1), (256, 1, 1), (300, 7, 2), (48, 40, 4), (600, 37, 8),
+                      (70, 1000, 1), (513, 9, 16)]:
+        n = nx * ny
         v = rng.normal(size=n) * 10.0 ** rng.integers(-3, 3, size=n)
-        P, _ = tiny_params(O)
+        P, _ = tiny_params(O, nx=nx, ny=ny)
         P.red_mode, P.red_k = O.RED_BLOCKED, k
         got = O.osum(P, v)
         # explicit restatement
-        cpb = 256 * k
-        nb = max(1, -(-n // cpb))
+        tiles_x = -(-nx // 256)
+        tiles_y = -(-ny // k)
         parts = []
-        for b in range(nb):
+        for b in range(tiles_x * tiles_y):
+            tx, y0 = b % tiles_x, (b // tiles_x) * k
             acc = []
             for t in range(256):
+                x = tx * 256 + t
                 s = 0.0
-                for j in range(k):
-                    i = b * cpb + j * 256 + t
-                    if i < n:
-                        s += float(v[i])
+                if x < nx:
+                    for y in range(y0, min(y0 + k, ny)):
+                        s += float(v[y * nx + x])
                 acc.append(s)
             parts.append(tree(acc))
         acc = []
         for t in range(256):
             s = 0.0
-            for j in range(t, nb, 256):
+            for j in range(t, len(parts), 256):
                 s += parts[j]
             acc.append(s)
         assert got == tree(acc)
