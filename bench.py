@@ -12,7 +12,11 @@ Prints one JSON line:
   metric  Mcell-steps/s = nx*ny*accepted steps / wall seconds of the timed loop / 1e6
   roofline  the dominant kernel's algorithmic HBM bytes per launch / its
             average launch time (HIP events on the library's stream over the
-            timed region), against 8 TB/s
+            timed region, the kernel alone), against 8 TB/s; `traffic` is the
+            kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE and
+            WRITE_SIZE scaled by a 1 GiB copy kernel) from the committed
+            counter summary of the same workload and tile height, when there
+            is one
   cpu_baseline  the reference solver compiled from source (or the oracle
             restatement when that is absent) on a bounded sample of the same
             workload on this host's cores
@@ -31,6 +35,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "fe-c_eutectoid_phase-field_simulation_amd")
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def load_pkg():
@@ -279,6 +284,7 @@ def main():
             log(f"  {name:16s} {v['launches']:6d} launches {v['ms']:9.3f} ms  "
                 f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s")
     mcs = nx * ny * args.steps / wall / 1e6
+    rows_per_tile = s.layout()[1]
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     roof = None
     if dom[1]["ms"] > 0:
@@ -287,6 +293,13 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": dom[1]["bytes"] / dom[1]["launches"],
                 "avg_launch_ms": dom[1]["ms"] / dom[1]["launches"]}
+        measured = os.path.join(PROFILES, f"hbm_traffic_fp64_{nx}x{ny}_k{rows_per_tile}.json")
+        if os.path.exists(measured):
+            with open(measured) as fh:
+                tr = json.load(fh)["kernels"].get(dom[0])
+            if tr:
+                roof["traffic"] = round(tr["traffic_bytes"])
+                roof["traffic_source"] = os.path.relpath(measured, ROOT)
     total_bytes = sum(v["bytes"] for v in stats.values())
     out = {
         "metric": "Mcell-steps/s", "value": round(mcs, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -295,7 +308,7 @@ def main():
         "config": {"workload": f"FP64 plain form, {nx}x{ny} lamellar colony ({args.pairs} pairs, +/-5% spacing, "
                                f"seed {args.seed}), {args.undercooling} K undercooling, adaptive dt, implicit "
                                f"diffusion, moving window",
-                   "nx": nx, "ny": ny, "precision": "fp64"},
+                   "nx": nx, "ny": ny, "precision": "fp64", "cells_per_thread": rows_per_tile},
         "attempts": c1["attempts"] - c0["attempts"],
         "rejected_attempts": c1["rejected_attempts"] - c0["rejected_attempts"],
         "cg_iterations": c1["cg_iterations"] - c0["cg_iterations"],
