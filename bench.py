@@ -137,6 +137,92 @@ def build_ensemble(pkg, count, seed, tmpdir):
     return doms
 
 
+def run_mixed_accuracy(args, pkg):
+    """FP32-mixed form against the FP64 plain form on the same inputs, in
+    lockstep: each step the FP64 context advances, then the mixed context
+    takes a step of exactly the accepted dt (so both follow one dt sequence
+    and the difference is precision alone).  Every --acc-every steps the
+    fields are compared: max |dmu| in J/mol and max |dphi| per phase
+    fraction.  Afterwards both forms run free from their states for the
+    timed throughput comparison."""
+    import numpy as np
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:  # noqa: BLE001
+        torch, have_torch = None, False
+    nx, ny = (8192, 4096) if (args.nx, args.ny) == (4096, 2048) else (args.nx, args.ny)
+    pairs = 64 if args.pairs == 32 else args.pairs
+    uc = 20.0 if args.undercooling == 10.0 else args.undercooling
+    tmp = tempfile.mkdtemp()
+    path, cfg, phi, c, mu = build_workload(pkg, nx, ny, pairs, ny // 8, args.seed, uc, tmp)
+    log(f"mixed accuracy: {nx}x{ny}, {pairs} pairs, {uc} K, {args.acc_steps} lockstep steps, compared every "
+        f"{args.acc_every}")
+    a = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=pkg.FP64)
+    if args.acc_other == "mixed":
+        b = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread,
+                       precision=pkg.MIXED)
+    else:
+        b = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=8, precision=pkg.FP64)
+    for s in (a, b):
+        s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
+    m0 = a.total_carbon()
+    rows, broken = [], None
+    for k in range(1, args.acc_steps + 1):
+        info = a.advance(1.0)
+        sc = b.scalars()
+        sc.dt = info.accepted_dt
+        b.set_scalars(sc)
+        r0 = b.counters()["rejected_attempts"]
+        b.advance(1.0)
+        if broken is None and (b.counters()["rejected_attempts"] != r0 or b.scalars().time != a.scalars().time):
+            broken = k
+        if k % args.acc_every == 0 or k == args.acc_steps:
+            pa, ca, ma, _ = a.get_state()
+            pb, cb, mb, _ = b.get_state()
+            row = {"step": k, "time": a.scalars().time, "max_dmu_J_per_mol": float(np.max(np.abs(ma - mb))),
+                   "max_dphi": [float(np.max(np.abs(pa[i] - pb[i]))) for i in range(3)],
+                   "max_dc": float(np.max(np.abs(ca - cb))),
+                   "mass_rel_mixed": abs(b.total_carbon() - m0) / abs(m0)}
+            rows.append(row)
+            log(f"  step {k}: t={row['time']:.4e} max|dmu| {row['max_dmu_J_per_mol']:.3e} J/mol, "
+                f"max|dphi| {row['max_dphi']}")
+    # free-running throughput of each form from its own state
+    timing = {}
+    for name, s in (("fp64", a), ("mixed" if args.acc_other == "mixed" else "other", b)):
+        s.run(1.0, 5)
+        c0 = s.counters()
+        if have_torch:
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = s.run(1.0, args.steps)
+        wall = time.perf_counter() - t0
+        c1 = s.counters()
+        timing[name] = {"value": nx * ny * r["taken"] / wall / 1e6, "steps": r["taken"], "wall_s": wall,
+                        "attempts": c1["attempts"] - c0["attempts"],
+                        "cg_iterations": c1["cg_iterations"] - c0["cg_iterations"]}
+        log(f"  free run {name}: {r['taken']} steps in {wall:.3f} s -> {timing[name]['value']:.1f} Mcell-steps/s")
+    a.close()
+    b.close()
+    other = "mixed" if args.acc_other == "mixed" else "other"
+    vmx, v64 = timing[other]["value"], timing["fp64"]["value"]
+    out = {
+        "metric": "Mcell-steps/s", "value": round(vmx, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": timing[other]["steps"], "warmup": 5,
+        "ms_per_step": round(timing[other]["wall_s"] * 1e3 / max(timing[other]["steps"], 1), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{'FP32-mixed form' if args.acc_other == 'mixed' else 'FP64 with 8-row tiles'} "
+                               f"against FP64 on {nx}x{ny}, {pairs} pairs +/-5% (seed {args.seed}), "
+                               f"{uc} K; errors in lockstep (one dt sequence)",
+                   "nx": nx, "ny": ny, "precision": args.acc_other},
+        "fp64_value": round(v64, 3), "mixed_over_fp64": round(vmx / v64, 3), "timing": timing,
+        "lockstep_steps": args.acc_steps, "lockstep_broken_at": broken, "comparisons": rows,
+        "max_dmu_J_per_mol": max(r["max_dmu_J_per_mol"] for r in rows),
+        "max_dphi": [max(r["max_dphi"][i] for r in rows) for i in range(3)],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_ensemble(args, pkg):
     """Batched ensemble in one context versus the same domains one at a time."""
     import numpy as np
@@ -149,8 +235,9 @@ def run_ensemble(args, pkg):
     doms = build_ensemble(pkg, args.domains, args.seed, tmp)
     cells = sum(d["cfg"].nx * d["cfg"].ny for d in doms)
     log(f"ensemble: {len(doms)} domains, {cells / 1e6:.2f} M cells")
+    precision = pkg.MIXED if args.precision == "mixed" else pkg.FP64
     b = pkg.Batch([d["cfg"].solver_params() for d in doms], [d["cfg"].initial_mu for d in doms],
-                  cells_per_thread=args.cells_per_thread)
+                  cells_per_thread=args.cells_per_thread, precision=precision)
     for i, d in enumerate(doms):
         b.set_domain_state(i, d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0])
     stops = [1e-6] * len(doms)
@@ -170,7 +257,8 @@ def run_ensemble(args, pkg):
     seq_steps = max(1, args.steps // 4)
     t_seq = 0.0
     for d in doms:
-        s = pkg.Solver(d["cfg"].solver_params(), d["cfg"].initial_mu)
+        s = pkg.Solver(d["cfg"].solver_params(), d["cfg"].initial_mu, cells_per_thread=args.cells_per_thread,
+                       precision=precision)
         s.set_state(d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0])
         s.run(1e-6, args.warmup)
         t0 = time.perf_counter()
@@ -182,8 +270,10 @@ def run_ensemble(args, pkg):
     out = {
         "metric": "Mcell-steps/s", "value": round(batched, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batched ensemble of {len(doms)} uneven domains (seed {args.seed}), FP64, "
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32", "data": "synthetic",
+        "config": {"workload": f"batched ensemble of {len(doms)} uneven domains (seed {args.seed}), "
+                               f"{'FP64' if args.precision == 'fp64' else 'FP32-mixed'}, "
                                f"one context, rounds replayed from a captured graph",
                    "domains": len(doms), "cells": cells, "cells_per_thread": args.cells_per_thread},
         "sequential_same_domains": round(sequential, 3), "batched_over_sequential": round(batched / sequential, 2),
@@ -205,7 +295,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--workload", choices=["production", "ensemble"], default="production")
+    ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy"], default="production")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: Jacobi diagonal and PCG vectors p, z stored in FP32 (state, operator "
+                         "coefficients, residuals, dots, mass and acceptance FP64)")
+    ap.add_argument("--acc-steps", type=int, default=3000, help="mixed-accuracy: FP64 steps in total")
+    ap.add_argument("--acc-every", type=int, default=1000, help="mixed-accuracy: steps between comparisons")
+    ap.add_argument("--acc-other", choices=["mixed", "fp64-k8"], default="mixed",
+                    help="mixed-accuracy: the form compared with FP64 at the default tile height; fp64-k8 is the "
+                         "FP64 form with another summation order (8-row tiles), the noise floor of the comparison")
     ap.add_argument("--domains", type=int, default=64)
     ap.add_argument("--cells-per-thread", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -221,6 +319,9 @@ def main():
     if args.workload == "ensemble":
         run_ensemble(args, pkg)
         return
+    if args.workload == "mixed-accuracy":
+        run_mixed_accuracy(args, pkg)
+        return
     try:
         import torch
         have_torch = torch.cuda.is_available()
@@ -232,7 +333,8 @@ def main():
     tmp = tempfile.mkdtemp()
     path, cfg, phi, c, mu = build_workload(pkg, nx, ny, args.pairs, strip_rows, args.seed, args.undercooling, tmp)
     log(f"workload {nx}x{ny}: {args.pairs} pairs, colony {strip_rows} rows, {args.undercooling} K")
-    s = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread)
+    precision = pkg.MIXED if args.precision == "mixed" else pkg.FP64
+    s = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=precision)
     s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
     initial_carbon = s.total_carbon()
     s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, initial_carbon, 0.0, 0.0, 0, 0, 0])
@@ -293,7 +395,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": dom[1]["bytes"] / dom[1]["launches"],
                 "avg_launch_ms": dom[1]["ms"] / dom[1]["launches"]}
-        measured = os.path.join(PROFILES, f"hbm_traffic_fp64_{nx}x{ny}_k{rows_per_tile}.json")
+        measured = os.path.join(PROFILES, f"hbm_traffic_{args.precision}_{nx}x{ny}_k{rows_per_tile}.json")
         if os.path.exists(measured):
             with open(measured) as fh:
                 tr = json.load(fh)["kernels"].get(dom[0])
@@ -304,11 +406,13 @@ def main():
     out = {
         "metric": "Mcell-steps/s", "value": round(mcs, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"FP64 plain form, {nx}x{ny} lamellar colony ({args.pairs} pairs, +/-5% spacing, "
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{'FP64 plain' if args.precision == 'fp64' else 'FP32-mixed'} form, {nx}x{ny} "
+                               f"lamellar colony ({args.pairs} pairs, +/-5% spacing, "
                                f"seed {args.seed}), {args.undercooling} K undercooling, adaptive dt, implicit "
                                f"diffusion, moving window",
-                   "nx": nx, "ny": ny, "precision": "fp64", "cells_per_thread": rows_per_tile},
+                   "nx": nx, "ny": ny, "precision": args.precision, "cells_per_thread": rows_per_tile},
         "attempts": c1["attempts"] - c0["attempts"],
         "rejected_attempts": c1["rejected_attempts"] - c0["rejected_attempts"],
         "cg_iterations": c1["cg_iterations"] - c0["cg_iterations"],

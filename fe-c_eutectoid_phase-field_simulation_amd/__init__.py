@@ -31,6 +31,7 @@ ERR_UNSAFE_SHIFT, ERR_NONFINITE_CARBON, ERR_INVALID_SHIFT, ERR_ALLOC, ERR_ARGUME
 WHY = ["phase increment", "linear/closure", "composition/phase bounds", "mass conservation",
        "free-energy increase"]
 FP64 = 0
+MIXED = 1  # Jacobi diagonal, p, z in FP32; state, operator, residuals, dots and acceptance FP64
 
 KERNELS = ["propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
            "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"]
@@ -98,12 +99,18 @@ class Diag(C.Structure):
 _lib = None
 
 # Every symbol declared in include/pearl_gpu.h.
+class SmoothReport(C.Structure):
+    _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("energy_initial", C.c_double),
+                ("energy_final", C.c_double), ("final_relative_energy_change", C.c_double),
+                ("final_max_phase_change", C.c_double)]
+
+
 EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state", "pearl_get_state",
            "pearl_advance", "pearl_run", "pearl_maybe_shift", "pearl_diagnostics", "pearl_propose_phi",
            "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
-           "pearl_batch_run"]
+           "pearl_batch_run", "pearl_smooth"]
 
 
 def lib():
@@ -126,6 +133,7 @@ def lib():
                                 C.POINTER(C.c_int), C.POINTER(StepInfo)]
         L.pearl_maybe_shift.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
         L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
+        L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
         L.pearl_propose_phi.argtypes = [vp, C.c_double, _dp, _dp]
         L.pearl_implicit_diffusion.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.POINTER(DiffusionInfo)]
         L.pearl_reduced_energy.argtypes = [vp, _dp]
@@ -226,6 +234,13 @@ class Solver:
         self._check(lib().pearl_set_state(self._h, _ptr(phi), _ptr(carbon), _ptr(mu),
                                           C.byref(sc) if sc is not None else None))
 
+    def set_scalars(self, scalars):
+        """Replace the State scalars only (null field pointers leave the
+        fields and the cached e0/mass0 untouched), as the host driver does
+        after the clock snap (pearl_serial.cpp:2309-2310)."""
+        sc = scalars if isinstance(scalars, Scalars) else Scalars.from_list(scalars)
+        self._check(lib().pearl_set_state(self._h, None, None, None, C.byref(sc)))
+
     def get_state(self):
         phi = np.empty((3, self.ny, self.nx))
         c = np.empty((self.ny, self.nx))
@@ -263,6 +278,15 @@ class Solver:
         d = Diag()
         self._check(lib().pearl_diagnostics(self._h, C.byref(d)))
         return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    def smooth(self, max_steps: int = 100, min_steps: int = 100, energy_rtol: float = 1e-10,
+               max_phase_change: float = 0.02) -> dict:
+        """Capillarity-only smoothing of the current phases in place
+        (smooth_interface_only, pearl_serial.cpp:869-958; defaults :1720-1723)."""
+        r = SmoothReport()
+        self._check(lib().pearl_smooth(self._h, int(max_steps), int(min_steps), float(energy_rtol),
+                                       float(max_phase_change), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in SmoothReport._fields_}
 
     # single reference functions ------------------------------------------------
     def propose_phi(self, dt: float):

@@ -30,13 +30,17 @@ const char* kKernelNames[PEARL_KERNEL_COUNT] = {
     "propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
     "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"};
 
-// Algorithmic FP64 words per cell of each kernel (each logical array counted
+// Algorithmic bytes per cell of each kernel (each logical array counted
 // once per pass): propose R phi(3),mu W phi'(3); prepare R phi'(3),c,mu
 // W chi,icpt,K,rhs,x,diag,r,z; apply R z,p_old,K,chi W p,Ap; update
 // R x,p,r,Ap,diag W x,r,z; true residual R x,K,chi,rhs,diag W r,z;
 // finalize+accept R phi'(3),x,K,c,icpt,chi W c',mu'; state reduce
-// R phi(3),c,mu.
-const double kWords[PEARL_KERNEL_COUNT] = {7, 13, 6, 8, 7, 0, 10, 5, 0, 0};
+// R phi(3),c,mu.  Plain form: 8 B per value.  Mixed form: diag, z and p are
+// 4 B.
+const double kBytes[2][PEARL_KERNEL_COUNT] = {
+    {56, 104, 48, 64, 56, 0, 80, 40, 0, 0},
+    {56, 96, 36, 52, 48, 0, 80, 40, 0, 0},
+};
 
 // Device run counter of each kernel id (-1: none).
 const int kRunOf[PEARL_KERNEL_COUNT] = {RUN_PROPOSE,       RUN_PREPARE, RUN_CG_APPLY, RUN_CG_UPDATE,
@@ -48,6 +52,7 @@ struct pearl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int ndom = 0, k = 1;
+    int mx = 0;  // mixed precision (PEARL_MIXED)
     std::vector<pearl_params> prm;
     std::vector<KP> kp;             // host copy of the parameter blocks
     std::vector<pearl_scalars> sc;  // host-side copy of the State scalars
@@ -108,6 +113,23 @@ int hip_fail(pearl_ctx* c, hipError_t e, const char* where) {
         }                                                                                                \
     } while (0)
 
+// Kernels whose PCG storage depends on the precision mode.
+#define PEARL_LAUNCH_K(ctx, kname, MX, grid, ...)                                                             \
+    do {                                                                                                      \
+        switch ((ctx)->k) {                                                                                   \
+            case 1: hipLaunchKernelGGL((kname<1, MX>), grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;   \
+            case 2: hipLaunchKernelGGL((kname<2, MX>), grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;   \
+            case 4: hipLaunchKernelGGL((kname<4, MX>), grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;   \
+            case 16: hipLaunchKernelGGL((kname<16, MX>), grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((kname<8, MX>), grid, dim3(TPB), 0, (ctx)->stream, __VA_ARGS__); break;  \
+        }                                                                                                     \
+    } while (0)
+#define PEARL_LAUNCH_P(ctx, kname, grid, ...)                          \
+    do {                                                               \
+        if ((ctx)->mx) PEARL_LAUNCH_K(ctx, kname, true, grid, __VA_ARGS__);  \
+        else PEARL_LAUNCH_K(ctx, kname, false, grid, __VA_ARGS__);          \
+    } while (0)
+
 hipEvent_t get_event(pearl_ctx* c) {
     if (!c->free_events.empty()) {
         hipEvent_t e = c->free_events.back();
@@ -154,12 +176,12 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
        [&] { hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B); });
     ev(PEARL_KERNEL_PROPOSE, [&](int s) { PEARL_LAUNCH(c, k_propose, g, B, s); },
        [&] { hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B); });
-    ev(PEARL_KERNEL_PREPARE, [&](int s) { PEARL_LAUNCH(c, k_prepare, g, B, s); },
+    ev(PEARL_KERNEL_PREPARE, [&](int s) { PEARL_LAUNCH_P(c, k_prepare, g, B, s); },
        [&] { hipLaunchKernelGGL(k_prepare_fin, gf, tb, 0, st, B); });
     for (int j = 0; j < slots; ++j) {
-        ev(PEARL_KERNEL_CG_APPLY, [&](int s) { PEARL_LAUNCH(c, k_cg_a, g, B, s); },
+        ev(PEARL_KERNEL_CG_APPLY, [&](int s) { PEARL_LAUNCH_P(c, k_cg_a, g, B, s); },
            [&] { hipLaunchKernelGGL(k_cg_a_fin, gf, tb, 0, st, B); });
-        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { PEARL_LAUNCH(c, k_cg_b, g, B, s); },
+        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { PEARL_LAUNCH_P(c, k_cg_b, g, B, s); },
            [&] { hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B); });
     }
     ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, B, s); },
@@ -205,7 +227,7 @@ void harvest(pearl_ctx* c) {
             if (r < 0) continue;
             const unsigned now = c->hS[d].runs[r];
             unsigned& seen = c->runs_seen[static_cast<size_t>(d) * NKIND + r];
-            c->bytes[kid] += kWords[kid] * 8.0 * c->kp[d].n * static_cast<double>(now - seen);
+            c->bytes[kid] += kBytes[c->mx][kid] * c->kp[d].n * static_cast<double>(now - seen);
             if (!c->prof) c->launches[kid] += now - seen;
             seen = now;
         }
@@ -446,12 +468,11 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
                        int cells_per_thread, pearl_ctx** out) {
     if (!out || count < 1 || !params || !mu_references) return PEARL_ERR_ARGUMENT;
     *out = nullptr;
-    if (precision != PEARL_FP64) return PEARL_ERR_ARGUMENT;
-    long long cells = 0;
-    for (int d = 0; d < count; ++d) {
+    if (precision != PEARL_FP64 && precision != PEARL_MIXED) return PEARL_ERR_ARGUMENT;
+    for (int d = 0; d < count; ++d)
         if (!params_ok(&params[d])) return PEARL_ERR_ARGUMENT;
-        cells += static_cast<long long>(params[d].nx) * params[d].ny;
-    }
+    // rows per tile; 4 measured best on the 4096x2048 workload (2, 8, 16
+    // within 2-7 % of it)
     int k = cells_per_thread;
     if (k <= 0) k = 4;
     if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return PEARL_ERR_ARGUMENT;
@@ -460,6 +481,7 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
     c->device = device;
     c->ndom = count;
     c->k = k;
+    c->mx = precision == PEARL_MIXED ? 1 : 0;
     c->prm.assign(params, params + count);
     c->sc.assign(count, pearl_scalars{});
     c->runs_seen.assign(static_cast<size_t>(count) * NKIND, 0u);
@@ -845,6 +867,104 @@ int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, d
     if (mu_out) HIP_CHECK(c, hipMemcpyAsync(mu_out, dom_mu(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return PEARL_OK;
+}
+
+int pearl_smooth(pearl_ctx* c, int max_steps, int min_steps, double energy_rtol, double max_phase_change,
+                 pearl_smooth_report* rep) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (!rep) return PEARL_ERR_ARGUMENT;
+    std::memset(rep, 0, sizeof(*rep));
+    if (max_steps < 1 || min_steps < 0 || min_steps > max_steps || !(energy_rtol > 0) || !(max_phase_change > 0) ||
+        max_phase_change > .2)
+        return fail(c, PEARL_ERR_ARGUMENT, "invalid sharp-smoothing controls");
+    st = read_scalars(c);
+    if (st) return st;
+    const KP& P = c->kp[0];
+    const pearl_params& prm = c->prm[0];
+    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+    DevScal& h = c->hS[0];
+    const int cur0 = h.cur;
+    // interface energy of the current phases (:880)
+    PEARL_LAUNCH(c, k_iface_energy, grid, c->B, 0);
+    hipLaunchKernelGGL(k_iface_energy_fin, one, tb, 0, c->stream, c->B, 0);
+    st = read_scalars(c);
+    if (st) return st;
+    rep->energy_initial = h.sm_energy * P.dx * P.dy;
+    double previous = rep->energy_initial;
+    const double eps = prm.eps, sigma = prm.sigma;
+    const double base = .12 * std::min(prm.dx * prm.dx, prm.dy * prm.dy) / (eps * sigma);  // :884
+    int result = PEARL_OK;
+    for (int step = 1; step <= max_steps; ++step) {
+        PEARL_LAUNCH(c, k_smooth_force, grid, c->B, 0);
+        hipLaunchKernelGGL(k_smooth_force_fin, one, tb, 0, c->stream, c->B, 0);
+        st = read_scalars(c);
+        if (st) return st;
+        const double fmax = h.sm_fmax;
+        if (!std::isfinite(fmax)) {
+            result = fail(c, PEARL_ERR_SMOOTHING, "nonfinite smoothing force");
+            break;
+        }
+        if (fmax == 0) {
+            rep->steps = step - 1;
+            rep->converged = 1;
+            rep->final_relative_energy_change = 0;
+            rep->final_max_phase_change = 0;
+            break;
+        }
+        double scale = std::min(base, max_phase_change / fmax);
+        bool accepted = false;
+        double e_new = previous, change = 0;
+        for (int bt = 0; bt < 24 && !accepted; ++bt) {
+            PEARL_LAUNCH(c, k_smooth_trial, grid, c->B, 0, scale);
+            hipLaunchKernelGGL(k_smooth_trial_fin, one, tb, 0, c->stream, c->B, 0);
+            st = read_scalars(c);
+            if (st) return st;
+            if (h.sm_bad > 0) {
+                result = fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+                break;
+            }
+            change = h.sm_change;
+            e_new = h.sm_energy * P.dx * P.dy;
+            const double etol = 1e-14 * std::max({1.0, std::abs(previous), std::abs(rep->energy_initial)});
+            if (std::isfinite(e_new) && e_new <= previous + etol) accepted = true;
+            else scale *= .5;
+        }
+        if (result) break;
+        if (!accepted) {
+            result = fail(c, PEARL_ERR_SMOOTHING, "sharp smoothing could not find an energy-decreasing step");
+            break;
+        }
+        // the trial phases become current
+        h.cur ^= 1;
+        st = write_scalars(c);
+        if (st) return st;
+        const double rel = std::abs(previous - e_new) / std::max(std::abs(previous), 1e-300);
+        rep->steps = step;
+        rep->energy_final = e_new;
+        rep->final_relative_energy_change = rel;
+        rep->final_max_phase_change = change;
+        previous = e_new;
+        if (step >= min_steps && rel < energy_rtol && change < 1e-5) {
+            rep->converged = 1;
+            break;
+        }
+    }
+    if (rep->steps == 0) rep->energy_final = rep->energy_initial;
+    // carbon and mu stay where they were: follow the phases to the other buffer
+    if (h.cur != cur0) {
+        const size_t n = static_cast<size_t>(P.n);
+        HIP_CHECK(c, hipMemcpyAsync(dom_c(c, 0, h.cur), dom_c(c, 0, cur0), n * sizeof(double), hipMemcpyDeviceToDevice,
+                                    c->stream));
+        HIP_CHECK(c, hipMemcpyAsync(dom_mu(c, 0, h.cur), dom_mu(c, 0, cur0), n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    }
+    h.e0_valid = 0;
+    h.m0_valid = 0;
+    st = write_scalars(c);
+    if (st) return st;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return result;
 }
 
 int pearl_reduced_energy(pearl_ctx* c, double* energy) {

@@ -21,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace pearl_gpu {
 
 constexpr int TPB = 256;        // threads per block of every cell kernel
@@ -94,6 +96,8 @@ struct DevScal {
     double e_valid, e_mass, e_energy;
     double front, gmax, removed;
     double dmin_c, dmax_c, dmin_mu, dmax_mu, dsumerr, dmass, denergy;
+    // interface smoothing: max |force|, max phase change, energy, failure flag
+    double sm_fmax, sm_change, sm_energy, sm_bad;
     // StepInfo of the last accepted step
     double i_dt, i_change, i_residual, i_closure, i_energy_change;
     int i_cg, i_retries;
@@ -715,9 +719,31 @@ __global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
     S->phase = S->live ? PH_SOLVE : PH_DECIDE;
 }
 
-// Mixture and mu - mu_ref of one cell of the trial state.
+// Storage type of the Jacobi diagonal and the PCG vectors p and z: double in
+// the plain form; float in the mixed form, where every value is used
+// exactly as stored (promoted back to double).  These only steer the
+// iteration -- the diagonal is the preconditioner, p and z the search
+// direction, and x += alpha p, r -= alpha A p use the same stored p -- so the
+// mixed solve reaches the same FP64 residual tolerance on the same FP64
+// operator.  The operator's coefficients stay FP64: rounding chi moves
+// mu = mu_ref + (c - intercept)/chi by its relative error times
+// |mu - mu_ref| (1e-4 J/mol at FP32), and rounding K changes the face terms,
+// about 0.4 of the diagonal at production dt, which accumulated to
+// 3e-2 J/mol over 1000 steps.  x, r, Ap, rhs, the intercept and every dot
+// product stay FP64.
+template <bool MX>
+using Coef = typename std::conditional<MX, float, double>::type;
+
+template <bool MX>
+__device__ __forceinline__ double stored(double v) {
+    return static_cast<double>(static_cast<Coef<MX>>(v));
+}
+
+// Mixture, diffusivity as stored and mu - mu_ref of one cell of the trial
+// state.
 struct Prep {
     Mix m;
+    double k;
     double x;
     bool ok;
 };
@@ -729,8 +755,9 @@ struct Prep {
 // the west/east cell (the same expression, hence the same bits).  Sums
 // rhs.rhs, r.z, r.r and flags an invalid mixture; the finisher forms the
 // tolerance and evaluates the loop head for iteration 0.
-template <int TY>
+template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
+    using T = Coef<MX>;
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
     DevScal* S = B.S + w.d;
@@ -747,9 +774,9 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     double* __restrict__ K = B.K + P.off;
     double* __restrict__ rhs = B.rhs + P.off;
     double* __restrict__ x = B.x + P.off;
-    double* __restrict__ diag = B.diag + P.off;
+    T* __restrict__ diag = reinterpret_cast<T*>(B.diag) + P.off;
     double* __restrict__ r = B.r + P.off;
-    double* __restrict__ z = B.z + P.off;
+    T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     march<TY, Prep>(
@@ -759,6 +786,7 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
             double p[3];
             load_phi(phi_t, n, j, p);
             q.ok = mixture(P, p, q.m);
+            q.k = q.m.K;
             q.x = mu_old[j] - P.mu_ref;
             return q;
         },
@@ -766,9 +794,10 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
             if (!c.ok) v[3] = 1.0;
             const double ic = composition(P.mu_ref, c.m);
             const double b = c_old[i] - ic;
-            chi[i] = c.m.s;
+            const double ch = c.m.s;
+            chi[i] = ch;
             icpt[i] = ic;
-            K[i] = c.m.K;
+            K[i] = c.k;
             rhs[i] = b;
             x[i] = c.x;
             double qw[3], qe[3];
@@ -776,14 +805,14 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
             load_phi(phi_t, n, i + col.de, qe);
             const double xw = mu_old[i + col.dw] - P.mu_ref, xe = mu_old[i + col.de] - P.mu_ref;
             double wf[4];
-            face_weights(P, y, c.m.K, mixture_K(P, qw), mixture_K(P, qe), s.m.K, nn.m.K, wf);
-            const double ax = apply_vals(c.m.s, dt, wf, c.x, xw, xe, s.x, nn.x);
-            const double d = c.m.s + dt * (wf[0] + wf[1] + wf[2] + wf[3]);
+            face_weights(P, y, c.k, mixture_K(P, qw), mixture_K(P, qe), s.k, nn.k, wf);
+            const double ax = apply_vals(ch, dt, wf, c.x, xw, xe, s.x, nn.x);
+            const double d = stored<MX>(ch + dt * (wf[0] + wf[1] + wf[2] + wf[3]));
             const double rr = b - ax;
-            const double zz = rr / d;
-            diag[i] = d;
+            const double zz = stored<MX>(rr / d);
+            diag[i] = static_cast<T>(d);
             r[i] = rr;
-            z[i] = zz;
+            z[i] = static_cast<T>(zz);
             v[0] += b * b;
             v[1] += rr * zz;
             v[2] += rr * rr;
@@ -828,8 +857,9 @@ struct SolK {
 // for breakdown (:1070) and forms the step length (:1071).  In state
 // CG_TRUE: the true residual r = rhs - A x with z = r/diag (:1053-1062);
 // the finisher accepts (:1056) or restarts from it (:1060-1065).
-template <int TY>
+template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
+    using T = Coef<MX>;
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
     DevScal* S = B.S + w.d;
@@ -838,17 +868,21 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const double dt = S->dt;
     const double* __restrict__ chi = B.chi + P.off;
     const double* __restrict__ K = B.K + P.off;
-    double* __restrict__ z = B.z + P.off;
+    T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
     const Col col = column<TY>(P, w.lb, P.ny);
     if (state == CG_ITER) {
         record(B, slot, w.lb, 1);
         const int pi = S->pidx;
-        const double* __restrict__ po = (pi ? B.p1 : B.p0) + P.off;
-        double* __restrict__ pn = (pi ? B.p0 : B.p1) + P.off;
+        const T* __restrict__ po = reinterpret_cast<const T*>(pi ? B.p1 : B.p0) + P.off;
+        T* __restrict__ pn = reinterpret_cast<T*>(pi ? B.p0 : B.p1) + P.off;
         double* __restrict__ Ap = B.Ap + P.off;
         const bool fresh = S->restart != 0;
         const double beta = S->beta;
-        auto dir = [&](const int j) { return fresh ? z[j] : z[j] + beta * po[j]; };
+        // the direction as it is stored; A p is formed from exactly these values
+        auto dir = [&](const int j) {
+            const double zj = z[j];
+            return stored<MX>(fresh ? zj : zj + beta * static_cast<double>(po[j]));
+        };
         double v[1] = {0.0};
         march<TY, DirK>(
             P, col, [&](const int j) { return DirK{dir(j), K[j]}; },
@@ -857,7 +891,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
                 double wf[4];
                 face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
                 const double a = apply_vals(chi[i], dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
-                pn[i] = c.p;
+                pn[i] = static_cast<T>(c.p);
                 Ap[i] = a;
                 v[0] += c.p * a;
             });
@@ -866,7 +900,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         record(B, slot, w.lb, 2);
         const double* __restrict__ rhs = B.rhs + P.off;
         const double* __restrict__ x = B.x + P.off;
-        const double* __restrict__ diag = B.diag + P.off;
+        const T* __restrict__ diag = reinterpret_cast<const T*>(B.diag) + P.off;
         double* __restrict__ r = B.r + P.off;
         double v[2] = {0.0, 0.0};
         march<TY, SolK>(
@@ -876,9 +910,9 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
                 double wf[4];
                 face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
                 const double rr = rhs[i] - apply_vals(chi[i], dt, wf, c.x, x[iw], x[ie], s.x, nn.x);
-                const double zz = rr / diag[i];
+                const double zz = stored<MX>(rr / static_cast<double>(diag[i]));
                 r[i] = rr;
-                z[i] = zz;
+                z[i] = static_cast<T>(zz);
                 v[0] += rr * rr;
                 v[1] += rr * zz;
             });
@@ -926,28 +960,29 @@ __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
 // PCG kernel B, after a step of kernel A: x += alpha p; r -= alpha Ap;
 // z = r/diag; sums r.z and r.r; the finisher forms beta (:1077) and
 // evaluates the loop head of the next iteration.
-template <int TY>
+template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
+    using T = Coef<MX>;
     const Where w = where(B, -1);
     const KP& P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_ITER && S->pad == 1)) return;
     record(B, slot, w.lb, 1);
-    const double* __restrict__ p = (S->pidx ? B.p1 : B.p0) + P.off;
+    const T* __restrict__ p = reinterpret_cast<const T*>(S->pidx ? B.p1 : B.p0) + P.off;
     double* __restrict__ x = B.x + P.off;
     double* __restrict__ r = B.r + P.off;
-    double* __restrict__ z = B.z + P.off;
+    T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
     const double* __restrict__ Ap = B.Ap + P.off;
-    const double* __restrict__ diag = B.diag + P.off;
+    const T* __restrict__ diag = reinterpret_cast<const T*>(B.diag) + P.off;
     const double alpha = S->alpha;
     double v[2] = {0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     for_column<TY>(P, col, [&](const int i, const int) {
-        x[i] += alpha * p[i];
+        x[i] += alpha * static_cast<double>(p[i]);
         const double rr = r[i] - alpha * Ap[i];
-        const double zz = rr / diag[i];
+        const double zz = stored<MX>(rr / static_cast<double>(diag[i]));
         r[i] = rr;
-        z[i] = zz;
+        z[i] = static_cast<T>(zz);
         v[0] += rr * zz;
         v[1] += rr * rr;
     });
@@ -1366,6 +1401,199 @@ __global__ __launch_bounds__(TPB) void k_shift(Bufs B, int d, int rows, double c
     });
     if (bad_mix) atomicOr(&S->err, ERRF_MIXTURE);
     if (bad_c) atomicOr(&S->err, ERRF_NONFINITE_CARBON);
+}
+
+// ---------------------------------------------------------------------------
+// Capillarity-only interface smoothing (smooth_interface_only :869-958 with
+// interface_energy_only :822-841).  The force planes reuse three solver work
+// planes (r, z, Ap), which are idle outside a diffusion solve.
+// ---------------------------------------------------------------------------
+
+// Obstacle + capillary force of one cell minus its mean (:896-915); the
+// same expressions as the capillary part of propose_phi.
+__device__ __forceinline__ void smooth_force_cell(const KP& P, int y, const double (&p)[3], const double (&qw)[3],
+                                                  const double (&qe)[3], const double (&qs)[3],
+                                                  const double (&qn)[3], double (&f)[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+        f[a] = P.obs * (p[a1] + p[a2]) + P.tr * p[a1] * p[a2];
+    }
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir) {
+        if ((dir == 2 && y == 0) || (dir == 3 && y == P.ny - 1)) continue;
+        const double(&q)[3] = dir == 0 ? qw : (dir == 1 ? qe : (dir == 2 ? qs : qn));
+        const double dd = dir < 2 ? P.dx2 : P.dy2, rdd = dir < 2 ? P.rdx2 : P.rdy2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 3; ++b) {
+                const double wv = div_const(P.face_coef * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                f[a] += wv * q[b];
+                f[b] -= wv * q[a];
+            }
+    }
+    const double mean = div_const(f[0] + f[1] + f[2], 3.0, P.r3);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] = f[a] - mean;
+}
+
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_smooth_force(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const int n = P.n;
+    const double* __restrict__ phi = B.phi[S->cur] + 3 * P.off;
+    double* __restrict__ f0 = B.r + P.off;
+    double* __restrict__ f1 = B.z + P.off;
+    double* __restrict__ f2 = B.Ap + P.off;
+    double v[1] = {0.0};
+    const Col col = column<TY>(P, lb, P.ny);
+    march<TY, Phases>(
+        P, col,
+        [&](const int j) {
+            Phases q;
+            load_phi(phi, n, j, q.p);
+            return q;
+        },
+        [&](const int i, const int y, const Phases& s, const Phases& c, const Phases& nn) {
+            double qw[3], qe[3], f[3];
+            load_phi(phi, n, i + col.dw, qw);
+            load_phi(phi, n, i + col.de, qe);
+            smooth_force_cell(P, y, c.p, qw, qe, s.p, nn.p, f);
+            f0[i] = f[0];
+            f1[i] = f[1];
+            f2[i] = f[2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(f[a]));
+        });
+    block_partial<1, 1u>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_smooth_force_fin(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    double v[1];
+    if (final_reduce<1, 1u>(v, B.part + P.poff, P.nb)) B.S[d].sm_fmax = v[0];
+}
+
+// Trial phases of one cell: z = phi - scale*force projected onto the
+// simplex (:921-931); bad when z is not finite or the projection fails.
+struct Trial {
+    double p[3];   // projected trial phases
+    double chg;    // max_a |trial - phi|
+    bool bad;
+};
+__device__ __forceinline__ Trial smooth_trial_at(const KP& P, const double* __restrict__ phi, const double* f0,
+                                                 const double* f1, const double* f2, int n, int j, double scale) {
+    Trial t;
+    double p[3], z[3];
+    load_phi(phi, n, j, p);
+    z[0] = p[0] - scale * f0[j];
+    z[1] = p[1] - scale * f1[j];
+    z[2] = p[2] - scale * f2[j];
+    t.bad = !(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]));
+    if (!project_simplex(z, P.r3, t.p)) t.bad = true;
+    t.chg = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) t.chg = dmaxd(t.chg, __builtin_fabs(t.p[a] - p[a]));
+    return t;
+}
+
+// interface_energy_only of one cell (:826-838): the cell term, then the east
+// and (below the top row) north face terms.
+__device__ __forceinline__ double iface_cell(const KP& P, const double (&p)[3], const double (&qe)[3],
+                                             const double (&qn)[3], bool north) {
+    double v = P.obs * (p[0] * p[1] + p[0] * p[2] + p[1] * p[2]) + P.tr * p[0] * p[1] * p[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = a + 1; b < 3; ++b) {
+            const double z = div_const(p[a] * qe[b] - qe[a] * p[b], P.dx, P.rdx);
+            v += P.eps_sigma * z * z;
+        }
+    if (north) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 3; ++b) {
+                const double z = div_const(p[a] * qn[b] - qn[a] * p[b], P.dy, P.rdy);
+                v += P.eps_sigma * z * z;
+            }
+    }
+    return v;
+}
+
+// One trial of the backtracking search (:917-937): trial phases into the
+// other phi buffer, max change, failure flag and the trial's interface
+// energy.  The north neighbour's trial comes from the column march, the
+// east neighbour's is recomputed from phi and the force.
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_smooth_trial(Bufs B, int d, double scale) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const int n = P.n;
+    const int cur = S->cur;
+    const double* __restrict__ phi = B.phi[cur] + 3 * P.off;
+    double* __restrict__ out = B.phi[cur ^ 1] + 3 * P.off;
+    const double* f0 = B.r + P.off;
+    const double* f1 = B.z + P.off;
+    const double* f2 = B.Ap + P.off;
+    double v[3] = {0.0, 0.0, 0.0};  // max change, bad, energy
+    const Col col = column<TY>(P, lb, P.ny);
+    march<TY, Trial>(
+        P, col, [&](const int j) { return smooth_trial_at(P, phi, f0, f1, f2, n, j, scale); },
+        [&](const int i, const int y, const Trial&, const Trial& c, const Trial& nn) {
+            const Trial e = smooth_trial_at(P, phi, f0, f1, f2, n, i + col.de, scale);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) out[a * n + i] = c.p[a];
+            v[0] = dmaxd(v[0], c.chg);
+            if (c.bad) v[1] = 1.0;
+            v[2] += iface_cell(P, c.p, e.p, nn.p, y != P.ny - 1);
+        });
+    block_partial<3, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_smooth_trial_fin(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    double v[3];
+    if (!final_reduce<3, (1u | (1u << 2))>(v, B.part + P.poff, P.nb)) return;
+    S->sm_change = v[0];
+    S->sm_bad = v[1];
+    S->sm_energy = v[2];
+}
+
+// interface_energy_only of the current phases.
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_iface_energy(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    DevScal* S = B.S + d;
+    const int lb = blockIdx.x;
+    const int n = P.n;
+    const double* __restrict__ phi = B.phi[S->cur] + 3 * P.off;
+    double v[1] = {0.0};
+    const Col col = column<TY>(P, lb, P.ny);
+    march<TY, Phases>(
+        P, col,
+        [&](const int j) {
+            Phases q;
+            load_phi(phi, n, j, q.p);
+            return q;
+        },
+        [&](const int i, const int y, const Phases&, const Phases& c, const Phases& nn) {
+            double qe[3];
+            load_phi(phi, n, i + col.de, qe);
+            v[0] += iface_cell(P, c.p, qe, nn.p, y != P.ny - 1);
+        });
+    block_partial<1, 0u>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_iface_energy_fin(Bufs B, int d) {
+    const KP& P = B.kp[d];
+    double v[1];
+    if (final_reduce<1, 0u>(v, B.part + P.poff, P.nb)) B.S[d].sm_energy = v[0];
 }
 
 // Clears error bits (a kernel, not a memset, so it is safe in graphs).

@@ -41,6 +41,8 @@ extern "C" {
 #define PEARL_ERR_ALLOC 9            /* device allocation failed */
 #define PEARL_ERR_ARGUMENT 10        /* invalid argument (sizes, pointers, parameters) */
 #define PEARL_ERR_DEVICE 11          /* HIP runtime error */
+#define PEARL_ERR_SMOOTHING 12       /* "nonfinite smoothing force" :913 or
+                                        "sharp smoothing could not find an energy-decreasing step" :941 */
 
 /* Rejection reasons, the reference's "why" strings (:1132-1160). */
 #define PEARL_WHY_PHASE 0   /* "phase increment" */
@@ -51,6 +53,10 @@ extern "C" {
 
 /* Precision forms. */
 #define PEARL_FP64 0        /* plain form: FP64 throughout */
+#define PEARL_MIXED 1       /* mixed form: the Jacobi diagonal and the PCG vectors p and z stored
+                               in FP32 and used exactly as stored; phi, c, mu, the operator's
+                               coefficients (chi, K), x, r, Ap, every residual, dot product, the
+                               carbon mass and the acceptance reductions in FP64 */
 
 /* Model and numerical parameters: the reference's Domain (:617), Thermo
  * (:76-91), PhaseParameters (:521-532) and Controls (:767-773). */
@@ -89,6 +95,13 @@ typedef struct {
 typedef struct {
     double c_min, c_max, mu_min, mu_max, phi_sum_error, mass, energy;
 } pearl_diag;
+
+/* Result of the capillarity-only smoothing (SmoothReport :843-852, without
+ * the phase fractions). */
+typedef struct {
+    int steps, converged;
+    double energy_initial, energy_final, final_relative_energy_change, final_max_phase_change;
+} pearl_smooth_report;
 
 typedef struct pearl_ctx pearl_ctx;
 
@@ -141,6 +154,16 @@ int pearl_implicit_diffusion(pearl_ctx* ctx, const double* phi_trial, double dt,
 int pearl_reduced_energy(pearl_ctx* ctx, double* energy);
 int pearl_total_carbon(pearl_ctx* ctx, double* mass);
 int pearl_valid_state(pearl_ctx* ctx, int* valid);
+
+/* Capillarity-only smoothing of the current phases in place (replaces
+ * smooth_interface_only :869-958, which initialize runs on sharp FILL
+ * labels, :1798): projected gradient steps on interface_energy_only
+ * (:822-841) with up to 24 halvings of the step per iteration.  The
+ * interface energy is summed in the blocked order.  Carbon and mu are left
+ * as they are.  Errors: PEARL_ERR_ARGUMENT ("invalid sharp-smoothing
+ * controls"), PEARL_ERR_SMOOTHING, PEARL_ERR_PROJECTION. */
+int pearl_smooth(pearl_ctx* ctx, int max_steps, int min_steps, double energy_rtol, double max_phase_change,
+                 pearl_smooth_report* report);
 
 /* Blocked-reduction layout in use: threads per block (256) and cells per
  * thread k.  A sum over an nx-by-ny domain is: tiles of 256 columns by k

@@ -1,0 +1,102 @@
+"""The mixed-precision form (PEARL_MIXED) against the FP64 plain form.
+
+The mixed form stores the Jacobi diagonal and the PCG vectors p and z in
+FP32 and uses them exactly as stored; phi, c, mu, the operator's
+coefficients (chi, K), x, r, Ap, every residual and dot product, the carbon
+mass and the acceptance reductions stay FP64.  The solve therefore reaches
+the same FP64 residual tolerance on the same operator, and the flux update
+conserves carbon to rounding.
+
+Bars (stated in the tests): stepped in lockstep (the mixed run takes the
+FP64 run's accepted dt each step, so the difference is precision alone),
+max |dmu| <= 1e-5 J/mol and max |dphi| <= 1e-8 per phase fraction after 40
+steps; carbon stays conserved to 1e-12 relative; the diffusion closure
+stays below the reference's closure tolerance (1e-10).  Free-running
+adaptive runs can take different accept/reject decisions on marginal
+attempts and then differ by time-discretisation error; bench.py reports
+that separately."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+MU_TOL = 1e-5
+PHI_TOL = 1e-8
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(pkg, g, precision, k=0):
+    s = pkg.Solver(g["pd"], float(g["sc0"][9]), cells_per_thread=k, precision=precision)
+    s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+    return s
+
+
+def lockstep(a, b, steps):
+    """Advance the FP64 context a step, then the mixed context b with the
+    accepted dt of that step, so both follow one dt sequence and the
+    difference is the precision error alone.  Returns False when b rejected
+    the step (its attempt count ran ahead)."""
+    for _ in range(steps):
+        info = a.advance(1e9)
+        sc = b.scalars()
+        sc.dt = info.accepted_dt
+        b.set_scalars(sc)
+        r0 = b.counters()["rejected_attempts"]
+        b.advance(1e9)
+        if b.counters()["rejected_attempts"] != r0 or b.scalars().time != a.scalars().time:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("name", ["small_48x40", "colony_256x128"])
+def test_mixed_tracks_fp64(pkg, name):
+    g = load_golden(name)
+    say("mixed vs fp64 (lockstep)", name)
+    with make(pkg, g, pkg.FP64) as a, make(pkg, g, pkg.MIXED) as b:
+        m0 = a.total_carbon()
+        for block in range(4):
+            assert lockstep(a, b, 10)
+            pa, ca, ma, _ = a.get_state()
+            pb, cb, mb, _ = b.get_state()
+            dmu = float(np.max(np.abs(ma - mb)))
+            dphi = [float(np.max(np.abs(pa[i] - pb[i]))) for i in range(3)]
+            say(f"  step {a.scalars().step} t={a.scalars().time:.4e} max|dmu| {dmu:.3e} max|dphi| {dphi}")
+            assert dmu <= MU_TOL
+            assert max(dphi) <= PHI_TOL
+        # carbon: flux form, no window here -> the total stays the initial one
+        assert abs(b.total_carbon() - m0) <= 1e-12 * abs(m0)
+
+
+def test_mixed_diffusion_closure(pkg):
+    g = load_golden("small_48x40")
+    with make(pkg, g, pkg.MIXED) as s:
+        ch, phit = s.propose_phi(float(g["dt_probe"]))
+        info = s.implicit_diffusion(phit, float(g["dt_probe"]))
+    assert info["ok"] and info["closure"] <= 1e-10
+    with make(pkg, g, pkg.FP64) as s:
+        info64 = s.implicit_diffusion(phit, float(g["dt_probe"]))
+    assert float(np.max(np.abs(info["mu"] - info64["mu"]))) <= MU_TOL
+    assert abs(info["it"] - info64["it"]) <= 1
+
+
+def test_mixed_batch_equals_single(pkg):
+    """A mixed batch steps each domain exactly like a mixed single context."""
+    g = load_golden("small_48x40")
+    with make(pkg, g, pkg.MIXED) as s:
+        s.run(1e9, 12)
+        ref = s.get_state()
+    with pkg.Batch([g["pd"], g["pd"]], [float(g["sc0"][9])] * 2, precision=pkg.MIXED) as b:
+        for d in range(2):
+            b.set_domain_state(d, g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        b.run_all([1e9, 1e9], [12, 12])
+        for d in range(2):
+            got = b.get_domain_state(d)
+            for x, y in zip(got[:3], ref[:3]):
+                assert np.array_equal(x, y)

@@ -1,0 +1,85 @@
+"""Capillarity-only smoothing on the GPU (pearl_smooth) against the oracle's
+smooth_interface_only restatement (pearl_serial.cpp:869-958).
+
+The smoothed phases depend on summation order only through the accept and
+convergence decisions (energy comparisons): force, step scale and projection
+are per cell or exact maxima.  So the GPU result must equal the oracle run
+in the same blocked order bit for bit, report included, and, when every
+decision agrees, also equal the oracle in the reference's sequential order
+(the oracle's sequential smoothing is bitwise the reference's,
+tests/test_oracle_vs_ref.py)."""
+import importlib
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # nx, ny, lamella pairs, strip rows, cells per thread, smoothing steps
+    (48, 40, 2, 10, 4, 100),
+    (300, 64, 4, 16, 2, 60),
+    (96, 33, 3, 12, 1, 40),
+    (520, 96, 8, 24, 8, 30),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def sharp_state(pkg, nx, ny, pairs, strip, steps):
+    inputs = importlib.import_module("pearl_amd.inputs")
+    config = importlib.import_module("pearl_amd.config")
+    initial = importlib.import_module("pearl_amd.initial")
+    path = os.path.join(tempfile.mkdtemp(), "smooth.in")
+    inputs.colony_input(path, nx, ny, pairs, strip, 1e-9,
+                        overrides={"smooth_steps": steps, "smooth_min_steps": steps})
+    cfg = config.read_config(path)
+    return cfg, initial.sharp_labels(cfg.nx, cfg.ny, cfg.filling)
+
+
+@pytest.mark.parametrize("nx,ny,pairs,strip,k,steps", CASES)
+def test_smoothing_matches_oracle(pkg, O, nx, ny, pairs, strip, k, steps):
+    say("smooth", nx, ny, pairs, "k", k)
+    cfg, phi0 = sharp_state(pkg, nx, ny, pairs, strip, steps)
+    n = nx * ny
+    c0 = np.full(n, 0.03)
+    mu0 = np.full(n, cfg.initial_mu)
+    with pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=k) as s:
+        s.set_state(phi0.reshape(3, n), c0, mu0, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
+        rep = s.smooth(cfg.smooth_steps, cfg.smooth_min_steps, cfg.smooth_energy_rtol, cfg.smooth_max_phase_change)
+        phi_g, c_g, mu_g, _ = s.get_state()
+    # carbon and mu are carried through untouched
+    assert np.array_equal(c_g.reshape(-1), c0) and np.array_equal(mu_g.reshape(-1), mu0)
+    # oracle, same blocked order
+    phi_b = np.ascontiguousarray(phi0, dtype=np.float64).copy()
+    Pb = O.params_from_dict(cfg.solver_params(), red_mode=O.RED_BLOCKED, red_k=k)
+    rc, orep = O.smooth(Pb, phi_b, cfg.smooth_steps, cfg.smooth_min_steps, cfg.smooth_energy_rtol,
+                        cfg.smooth_max_phase_change)
+    assert rc == 0
+    assert np.array_equal(phi_g.reshape(3, ny, nx), phi_b)
+    assert rep["steps"] == int(orep[0]) and rep["converged"] == int(orep[1])
+    assert rep["energy_initial"] == orep[2] and rep["energy_final"] == orep[3]
+    assert rep["final_relative_energy_change"] == orep[4] and rep["final_max_phase_change"] == orep[5]
+    # oracle, the reference's sequential order
+    phi_s = np.ascontiguousarray(phi0, dtype=np.float64).copy()
+    Ps = O.params_from_dict(cfg.solver_params())
+    rc, srep = O.smooth(Ps, phi_s, cfg.smooth_steps, cfg.smooth_min_steps, cfg.smooth_energy_rtol,
+                        cfg.smooth_max_phase_change)
+    assert rc == 0 and int(srep[0]) == rep["steps"]
+    assert np.array_equal(phi_g.reshape(3, ny, nx), phi_s)
+    assert abs(rep["energy_final"] - srep[3]) <= 1e-12 * abs(srep[3])
+
+
+def test_smoothing_rejects_bad_controls(pkg):
+    cfg, phi0 = sharp_state(pkg, 48, 40, 2, 10, 10)
+    n = 48 * 40
+    with pkg.Solver(cfg.solver_params(), cfg.initial_mu) as s:
+        s.set_state(phi0.reshape(3, n), np.zeros(n), np.zeros(n), [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
+        for args in [(0, 0, 1e-10, 0.02), (10, 11, 1e-10, 0.02), (10, 10, 0.0, 0.02), (10, 10, 1e-10, 0.3)]:
+            with pytest.raises(pkg.PearlError, match="invalid sharp-smoothing controls"):
+                s.smooth(*args)
