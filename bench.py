@@ -236,8 +236,18 @@ def run_ensemble(args, pkg):
     cells = sum(d["cfg"].nx * d["cfg"].ny for d in doms)
     log(f"ensemble: {len(doms)} domains, {cells / 1e6:.2f} M cells")
     precision = pkg.MIXED if args.precision == "mixed" else pkg.FP64
-    b = pkg.Batch([d["cfg"].solver_params() for d in doms], [d["cfg"].initial_mu for d in doms],
-                  cells_per_thread=args.cells_per_thread, precision=precision)
+    params = [d["cfg"].solver_params() for d in doms]
+    mus = [d["cfg"].initial_mu for d in doms]
+    states = [(d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0]) for d in doms]
+    k = args.cells_per_thread
+    tuned = None
+    if k == 0:
+        import importlib
+        autotune = importlib.import_module("pearl_amd.autotune")
+        log("autotuning the tile height ...")
+        k, tuned = autotune.tune_batch(pkg, params, mus, states, precision=precision, log=log)
+        log(f"autotune: {k} rows per tile")
+    b = pkg.Batch(params, mus, cells_per_thread=k, precision=precision)
     for i, d in enumerate(doms):
         b.set_domain_state(i, d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0])
     stops = [1e-6] * len(doms)
@@ -257,8 +267,7 @@ def run_ensemble(args, pkg):
     seq_steps = max(1, args.steps // 4)
     t_seq = 0.0
     for d in doms:
-        s = pkg.Solver(d["cfg"].solver_params(), d["cfg"].initial_mu, cells_per_thread=args.cells_per_thread,
-                       precision=precision)
+        s = pkg.Solver(d["cfg"].solver_params(), d["cfg"].initial_mu, cells_per_thread=k, precision=precision)
         s.set_state(d["phi"], d["c"], d["mu"], [0.0, d["cfg"].dt, 0, 0, 0, 0, 0, 0, 0])
         s.run(1e-6, args.warmup)
         t0 = time.perf_counter()
@@ -275,7 +284,8 @@ def run_ensemble(args, pkg):
         "config": {"workload": f"batched ensemble of {len(doms)} uneven domains (seed {args.seed}), "
                                f"{'FP64' if args.precision == 'fp64' else 'FP32-mixed'}, "
                                f"one context, rounds replayed from a captured graph",
-                   "domains": len(doms), "cells": cells, "cells_per_thread": args.cells_per_thread},
+                   "domains": len(doms), "cells": cells, "cells_per_thread": k},
+        "autotune_Mcell_steps_per_s": {str(kk): round(v, 1) for kk, v in tuned.items()} if tuned else None,
         "sequential_same_domains": round(sequential, 3), "batched_over_sequential": round(batched / sequential, 2),
         "attempts": ctr["attempts"], "cg_iterations": ctr["cg_iterations"],
     }
