@@ -72,10 +72,20 @@ def build_workload(pkg, nx, ny, pairs, strip_rows, seed, undercooling, tmpdir):
 
 
 def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
-    """Time the reference (built from source) on the same state for about
-    `seconds`, in a child process (the reference library carries its own
-    HDF5 and C++ runtime and is never loaded next to the GPU runtime).  Falls
-    back to the oracle restatement when the reference build is absent."""
+    """Time the reference on the same state, in child processes (the
+    reference library carries its own HDF5 and C++ runtime and is never
+    loaded next to the GPU runtime):
+
+    - serial: the reference advance() (built from pearl_serial.cpp) for
+      about `seconds`;
+    - MPI: the unmodified pearl_parallel under mpiexec on min(16, host CPUs)
+      ranks, restarted from the same state written as a reference
+      checkpoint, for about the same wall time (step count from the serial
+      rate).
+
+    Returns (primary, serial): the MPI figure when it ran, else the serial
+    one.  Falls back to the oracle restatement when the reference build is
+    absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -103,9 +113,32 @@ def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
             el = time.perf_counter() - t0
             if el >= seconds:
                 break
-    return dict(value=n * steps / el / 1e6, unit="Mcell-steps/s", cores=1, kind=kind,
-                sample=f"{steps} accepted steps of the same {cfg.nx}x{cfg.ny} workload from the same initial "
-                       f"state, serial, {el:.1f} s")
+    serial = dict(value=n * steps / el / 1e6, unit="Mcell-steps/s", cores=1, kind=kind,
+                  sample=f"{steps} accepted steps of the same {cfg.nx}x{cfg.ny} workload from the same initial "
+                         f"state, serial ({'pearl_serial.cpp advance' if kind == 'reference' else 'oracle'}), "
+                         f"{el:.1f} s")
+    if kind != "reference" or not ref_tool.mpiexec() or not ref_tool.parallel_program():
+        return serial, serial
+    nprocs = max(1, min(16, os.cpu_count() or 1))
+    while nprocs > 1:
+        try:
+            ref_tool.grid_split(nprocs, cfg.nx, cfg.ny)
+            break
+        except ValueError:
+            nprocs -= 1
+    mpi_steps = max(4, int(steps / el * seconds * nprocs * 0.5))
+    out = os.path.join(tmpdir, "baseline_mpi.json")
+    try:
+        ref_tool.call("baseline-mpi", path, state, nprocs, mpi_steps, out, timeout=int(seconds * 40 + 300))
+    except Exception as e:  # noqa: BLE001
+        log(f"cpu baseline: pearl_parallel run failed ({e}); reporting the serial figure")
+        return serial, serial
+    r = json.load(open(out))
+    mpi = dict(value=n * r["steps"] / r["seconds"] / 1e6, unit="Mcell-steps/s", cores=nprocs, kind="reference",
+               sample=f"{r['steps']} accepted steps of the same {cfg.nx}x{cfg.ny} workload, restarted from the same "
+                      f"state, unmodified pearl_parallel under mpiexec -n {nprocs} ({r['px']}x{r['py']} ranks), "
+                      f"step-loop wall {r['seconds']:.1f} s from its diagnostics.csv")
+    return mpi, serial
 
 
 def build_ensemble(pkg, count, seed, tmpdir):
@@ -435,7 +468,8 @@ def main():
     s.close()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(path, cfg, phi, c, mu, initial_carbon, args.cpu_seconds, tmp)
+        out["cpu_baseline"], out["cpu_baseline_serial"] = cpu_baseline(path, cfg, phi, c, mu, initial_carbon,
+                                                                       args.cpu_seconds, tmp)
     print(json.dumps(out), flush=True)
 
 

@@ -10,6 +10,12 @@ TEST INFRASTRUCTURE ONLY (tests/ and the CPU-baseline leg of bench.py).
       ledger checks included) and save phi, carbon, mu, scalars
   python oracle/ref_tool.py baseline INPUT.in STATE.npz SECONDS OUT.json
       time the reference advance() from the saved state, serially
+  python oracle/ref_tool.py baseline-mpi INPUT.in STATE.npz NPROCS STEPS OUT.json
+      write the saved state as a reference checkpoint (the reference's own
+      writer), run the unmodified pearl_parallel under mpiexec on NPROCS
+      ranks from it for STEPS accepted steps, and report steps and the step
+      loop's wall time from its diagnostics.csv (wall_s of the snapshot
+      after the last step minus that of the snapshot before the first)
 """
 from __future__ import annotations
 
@@ -31,6 +37,36 @@ def call(*args, timeout=600):
     if r.returncode != 0:
         raise RuntimeError(f"ref_tool {args[0]} failed: {r.stderr.strip()[-2000:]}")
     return r.stdout
+
+
+def mpiexec():
+    for cand in ("/opt/conda/bin/mpiexec", "mpiexec"):
+        if os.path.isabs(cand) and os.path.exists(cand):
+            return cand
+    import shutil
+    return shutil.which("mpiexec")
+
+
+def parallel_program():
+    p = os.path.join(HERE, "_ref", "pearl_parallel")
+    return p if os.path.exists(p) else None
+
+
+def grid_split(nprocs, nx, ny):
+    """px*py = nprocs with px | nx and py | ny, as square as possible."""
+    best = None
+    for px in range(1, nprocs + 1):
+        if nprocs % px:
+            continue
+        py = nprocs // px
+        if nx % px or ny % py:
+            continue
+        score = abs((nx / px) - (ny / py))
+        if best is None or score < best[0]:
+            best = (score, px, py)
+    if best is None:
+        raise ValueError(f"no {nprocs}-rank split of {nx}x{ny}")
+    return best[1], best[2]
 
 
 def read_checkpoint(inp, h5, tmpdir):
@@ -69,6 +105,31 @@ def main(argv):
                 break
         with open(out, "w") as f:
             json.dump({"steps": steps, "seconds": el}, f)
+        return 0
+    if cmd == "baseline-mpi":
+        import tempfile
+        inp, state, nprocs, steps, out = argv[1], argv[2], int(argv[3]), int(argv[4]), argv[5]
+        z = np.load(state)
+        R = O.Ref(inp)
+        R.set_state(z["phi"], z["carbon"], z["mu"], z["scalars"])
+        tmp = tempfile.mkdtemp()
+        ckpt = os.path.join(tmp, "start.h5")
+        R.write_frame(ckpt)
+        ny, nx = z["carbon"].shape
+        px, py = grid_split(nprocs, nx, ny)
+        run_dir = os.path.join(tmp, "run")
+        cmd = [mpiexec(), "-n", str(nprocs), parallel_program(), inp, run_dir, "--restart", ckpt,
+               "--max-steps", str(steps), "--px", str(px), "--py", str(py)]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0:
+            raise RuntimeError(f"pearl_parallel failed: {r.stdout[-1000:]} {r.stderr[-2000:]}")
+        with open(os.path.join(run_dir, "diagnostics.csv")) as f:
+            rows = [line.strip().split(",") for line in f if line.strip()]
+        head, first, last = rows[0], rows[1], rows[-1]
+        si, wi = head.index("step"), head.index("wall_s")
+        with open(out, "w") as f:
+            json.dump({"steps": int(last[si]) - int(first[si]), "seconds": float(last[wi]) - float(first[wi]),
+                       "nprocs": nprocs, "px": px, "py": py}, f)
         return 0
     raise SystemExit(f"unknown command {cmd}")
 
