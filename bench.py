@@ -53,22 +53,8 @@ def log(*a):
 
 def build_workload(pkg, nx, ny, pairs, strip_rows, seed, undercooling, tmpdir):
     import importlib
-    import numpy as np
     inputs = importlib.import_module("pearl_amd.inputs")
-    config = importlib.import_module("pearl_amd.config")
-    initial = importlib.import_module("pearl_amd.initial")
-    path = os.path.join(tmpdir, "workload.in")
-    fills = inputs.lamella_fills(nx, pairs, strip_rows, spacing_noise=0.05, seed=seed)
-    inputs.write_input(path, nx, ny, 1e-6, 3e-10, fills,
-                       overrides={"undercooling": repr(float(undercooling)),
-                                  "max_undercooling": repr(float(max(10.0, math.ceil(undercooling)))),
-                                  "moving_window": 1})
-    cfg = config.read_config(path)
-    phi = initial.diffuse_colony(nx, ny, cfg.dx, cfg.dy, cfg.eps, initial.plates_from_fills(fills), strip_rows)
-    mu0 = cfg.initial_mu
-    c = initial.carbon_at_mu(phi, cfg.thermo, mu0)
-    mu = np.full((ny, nx), mu0)
-    return path, cfg, phi, c, mu
+    return inputs.colony_workload(nx, ny, pairs, strip_rows, seed, undercooling, tmpdir)
 
 
 def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
@@ -325,6 +311,88 @@ def run_ensemble(args, pkg):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint(args, pkg):
+    """The checkpointed adjoint on the parameter-recovery case: 512x256, 4
+    lamella pairs, 15 K, a frozen dt sequence of --adj-steps steps,
+    checkpoints every --adj-every steps (torch_adjoint.colony_inverse_problem).
+    Times one pearl_adjoint call (forward with checkpoints, loss, reverse
+    sweep with segment recomputation) and the forward alone, and checks the
+    gradient against an FP64 autograd reference (the step restated in torch
+    with a stencil PCG solve differentiated implicitly, on the same GPU) and
+    against central differences of the forward at h = 1e-6 and 1e-8
+    relative."""
+    import importlib
+    import numpy as np
+    import torch
+    TA = importlib.import_module("pearl_amd.torch_adjoint")
+    nx, ny = (512, 256) if (args.nx, args.ny) == (4096, 2048) else (args.nx, args.ny)
+    pairs = 4 if args.pairs == 32 else args.pairs
+    uc = 15.0 if args.undercooling == 10.0 else args.undercooling
+    steps, every = args.adj_steps, args.adj_every
+    log(f"adjoint: {nx}x{ny}, {pairs} pairs, {uc} K, {steps} steps, checkpoints every {every}")
+    prob, factors, cfg = TA.colony_inverse_problem(nx, ny, pairs, uc, steps, every, seed=args.seed)
+    theta = prob.theta0().tolist()
+    prob.loss_and_grad(theta)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        th = prob.theta0().requires_grad_(True)
+        L = TA.phase_field_loss(th, prob)
+        L.backward()
+    wall = (time.perf_counter() - t0) / reps
+    grad = th.grad.numpy().copy()
+    t0 = time.perf_counter()
+    L_fwd = prob.loss(theta)
+    wall_fwd = time.perf_counter() - t0
+    log(f"  adjoint {wall:.3f} s per call, forward alone {wall_fwd:.3f} s; loss {L.item():.12e}")
+    fd = {}
+    for rh in (1e-6, 1e-8):
+        row = []
+        for k, name in enumerate(TA.THETA):
+            h = rh * theta[k]
+            tp, tm = list(theta), list(theta)
+            tp[k] += h
+            tm[k] -= h
+            d = (prob.loss(tp) - prob.loss(tm)) / (2 * h)
+            row.append({"fd": d, "rel": abs(d - grad[k]) / abs(grad[k])})
+        fd[f"{rh:.0e}"] = row
+        log(f"  central differences h={rh:.0e}: rel " + " ".join(f"{r['rel']:.1e}" for r in row))
+    ref = None
+    if not args.adj_no_reference:
+        import sys as _s
+        _s.path.insert(0, os.path.join(ROOT, "tests"))
+        import adjoint_reference as R
+        phi, c, mu, _ = prob.state
+        t0 = time.perf_counter()
+        Lr, gr, phr, *_ = R.loss_and_grad(prob.params, prob.mu_reference, phi, c, mu, prob.dts, prob.phi_target,
+                                          prob.w_front, prob.front_target, solve="cg", device="cuda", every=every)
+        t_ref = time.perf_counter() - t0
+        dphi = float(np.max(np.abs(prob.final_phases(theta) - phr)))
+        ref = {"seconds": round(t_ref, 2), "loss": Lr, "gradient": [float(v) for v in gr],
+               "rel": [float(abs(grad[k] - gr[k]) / abs(gr[k])) for k in range(7)],
+               "final_phases_max_diff": dphi,
+               "what": "the step restated in torch (FP64, stencil PCG solve differentiated implicitly, "
+                       f"checkpointed every {every} steps) on the same GPU, autograd"}
+        log(f"  torch reference {t_ref:.1f} s; rel " + " ".join(f"{v:.1e}" for v in ref["rel"]) +
+            f"; final phases differ by {dphi:.1e}")
+    out = {
+        "metric": "Mcell-steps/s", "value": round(nx * ny * steps / wall / 1e6, 3), "unit": "Mcell-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(wall * 1e3 / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"checkpointed adjoint (forward + reverse) of {steps} fixed steps on {nx}x{ny}, "
+                               f"{pairs} lamella pairs, {uc} K, checkpoints every {every}; target from sigma, "
+                               f"tau_ag, tau_tg scaled by seeded factors in [0.8, 1.2]",
+                   "nx": nx, "ny": ny, "every": every},
+        "factors": factors, "loss": L.item(), "loss_forward_only": L_fwd,
+        "theta": dict(zip(TA.THETA, theta)), "gradient": dict(zip(TA.THETA, [float(v) for v in grad])),
+        "forward_only_Mcell_steps_per_s": round(nx * ny * steps / wall_fwd / 1e6, 3),
+        "adjoint_over_forward_time": round(wall / wall_fwd, 2),
+        "central_differences": fd, "torch_reference": ref,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -338,7 +406,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
-    ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy"], default="production")
+    ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy", "adjoint"], default="production")
+    ap.add_argument("--adj-steps", type=int, default=2000, help="adjoint: fixed steps")
+    ap.add_argument("--adj-every", type=int, default=100, help="adjoint: checkpoint interval")
+    ap.add_argument("--adj-no-reference", action="store_true", help="adjoint: skip the torch autograd reference")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: Jacobi diagonal and PCG vectors p, z stored in FP32 (state, operator "
                          "coefficients, residuals, dots, mass and acceptance FP64)")
@@ -364,6 +435,9 @@ def main():
         return
     if args.workload == "mixed-accuracy":
         run_mixed_accuracy(args, pkg)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args, pkg)
         return
     try:
         import torch

@@ -110,7 +110,7 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
-           "pearl_batch_run", "pearl_smooth"]
+           "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint"]
 
 
 def lib():
@@ -134,6 +134,8 @@ def lib():
         L.pearl_maybe_shift.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
         L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
+        L.pearl_step_fixed.argtypes = [vp, C.c_double]
+        L.pearl_adjoint.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
         L.pearl_propose_phi.argtypes = [vp, C.c_double, _dp, _dp]
         L.pearl_implicit_diffusion.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.POINTER(DiffusionInfo)]
         L.pearl_reduced_energy.argtypes = [vp, _dp]
@@ -278,6 +280,22 @@ class Solver:
         d = Diag()
         self._check(lib().pearl_diagnostics(self._h, C.byref(d)))
         return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    def step_fixed(self, dt: float):
+        """One always-taken step of size dt (propose_phi, implicit_diffusion)."""
+        self._check(lib().pearl_step_fixed(self._h, float(dt)))
+
+    def adjoint(self, dts, phi_target, every: int = 100, w_front: float = 0.0, front_target: float = 0.0):
+        """Loss and gradient w.r.t. (sigma, tau_at, tau_ag, tau_tg, D_alpha,
+        D_theta, D_gamma) after fixed steps dts from the current state; see
+        pearl_adjoint.  The context ends on the final state."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        tgt = np.ascontiguousarray(phi_target, dtype=np.float64).reshape(3, self.ny, self.nx)
+        loss = C.c_double(0)
+        grad = np.zeros(7)
+        self._check(lib().pearl_adjoint(self._h, int(dts.size), _ptr(dts), int(every), _ptr(tgt), float(w_front),
+                                        float(front_target), C.byref(loss), _ptr(grad)))
+        return loss.value, grad
 
     def smooth(self, max_steps: int = 100, min_steps: int = 100, energy_rtol: float = 1e-10,
                max_phase_change: float = 0.02) -> dict:

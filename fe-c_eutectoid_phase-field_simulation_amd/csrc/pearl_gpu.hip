@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "pearl_adjoint.hpp"
 #include "pearl_kernels.hpp"
 
 using namespace pearl_gpu;
@@ -865,6 +866,254 @@ int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, d
     const int src = finalized ? (h.cur ^ 1) : h.cur;
     if (carbon_out) HIP_CHECK(c, hipMemcpyAsync(carbon_out, dom_c(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (mu_out) HIP_CHECK(c, hipMemcpyAsync(mu_out, dom_mu(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return PEARL_OK;
+}
+
+namespace {
+// One step of size dt that is always taken: propose_phi (:961-1002), then
+// implicit_diffusion (:1010-1094) on the trial phases, then the trial state
+// becomes current.  No acceptance test, no dt control: the differentiable
+// map that the adjoint inverts.  Domain 0, coupled mode, FP64.
+int step_fixed(pearl_ctx* c, double dt) {
+    int st = read_scalars(c);
+    if (st) return st;
+    DevScal& h = c->hS[0];
+    h.phase = PH_PROPOSE;
+    h.need_state = 0;
+    h.dt = dt;
+    h.status = ST_OK;
+    h.err = 0;
+    st = write_scalars(c);
+    if (st) return st;
+    PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
+    hipLaunchKernelGGL(k_propose_fin, dim3(static_cast<unsigned>(c->ndom)), dim3(TPB), 0, c->stream, c->B);
+    st = read_scalars(c);
+    if (st) return st;
+    if (h.status == ST_PROJECTION) {
+        h.phase = PH_IDLE;
+        h.status = ST_OK;
+        (void)write_scalars(c);
+        return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+    }
+    h.phase = PH_SOLVE;
+    h.live = 1;
+    h.cg_state = CG_IDLE;
+    h.cg_it = 0;
+    h.cg_res = 0;
+    h.closure = 0;
+    h.pad = 0;
+    st = write_scalars(c);
+    if (st) return st;
+    for (int guard = 0; guard < 100000; ++guard) {
+        enqueue_round(c, c->cg_slots, false, false, false);
+        st = read_scalars(c);
+        if (st) return st;
+        if (h.phase != PH_SOLVE) break;
+    }
+    if (h.status != ST_OK) {
+        const int code = status_of(c, 0);
+        h.phase = PH_IDLE;
+        h.status = ST_OK;
+        (void)write_scalars(c);
+        return code;
+    }
+    if (h.cg_state != CG_OK) {
+        h.phase = PH_IDLE;
+        (void)write_scalars(c);
+        return fail(c, PEARL_ERR_ARGUMENT, "fixed step: the linear solve did not converge");
+    }
+    h.cur ^= 1;
+    h.time = h.time + dt;
+    h.step = h.step + 1;
+    h.e0_valid = 0;
+    h.m0_valid = 0;
+    h.phase = PH_IDLE;
+    st = write_scalars(c);
+    if (st) return st;
+    c->sc[0].time = h.time;
+    c->sc[0].step = h.step;
+    return PEARL_OK;
+}
+}  // namespace
+
+int pearl_step_fixed(pearl_ctx* c, double dt) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (c->prm[0].phase_only || c->mx || !(dt > 0) || !std::isfinite(dt)) return PEARL_ERR_ARGUMENT;
+    return step_fixed(c, dt);
+}
+
+namespace {
+// Device copy of the five state planes (phi x3, c, mu) of domain 0.
+int copy_state(pearl_ctx* c, double* dst_phi, double* dst_c, double* dst_mu, const double* src_phi,
+               const double* src_c, const double* src_mu) {
+    const size_t n = static_cast<size_t>(c->kp[0].n) * sizeof(double);
+    HIP_CHECK(c, hipMemcpyAsync(dst_phi, src_phi, 3 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(dst_c, src_c, n, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(dst_mu, src_mu, n, hipMemcpyDeviceToDevice, c->stream));
+    return PEARL_OK;
+}
+
+// Adjoint PCG solve A lam = gx (A of the trial phases, dt of the step),
+// lam left in B.x.  Reuses the forward PCG kernels and their controller.
+int adjoint_solve(pearl_ctx* c, const Adj& A) {
+    DevScal& h = c->hS[0];
+    h.phase = PH_SOLVE;
+    h.cg_state = CG_IDLE;
+    h.cg_it = 0;
+    h.cg_res = 0;
+    h.pad = 0;
+    h.pidx = 0;
+    h.dt = A.dt;
+    h.status = ST_OK;
+    int st = write_scalars(c);
+    if (st) return st;
+    const KP& P = c->kp[0];
+    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+    PEARL_LAUNCH(c, k_adj_prepare, grid, c->B, A);
+    hipLaunchKernelGGL(k_adj_prepare_fin, one, tb, 0, c->stream, c->B);
+    for (int guard = 0; guard < 10000; ++guard) {
+        st = read_scalars(c);
+        if (st) return st;
+        if (h.cg_state == CG_OK) break;
+        if (h.cg_state == CG_FAIL) return fail(c, PEARL_ERR_ARGUMENT, "adjoint: the linear solve did not converge");
+        for (int k = 0; k < c->cg_slots; ++k) {
+            PEARL_LAUNCH_P(c, k_cg_a, grid, c->B, -1);
+            hipLaunchKernelGGL(k_cg_a_fin, one, tb, 0, c->stream, c->B);
+            PEARL_LAUNCH_P(c, k_cg_b, grid, c->B, -1);
+            hipLaunchKernelGGL(k_cg_b_fin, one, tb, 0, c->stream, c->B);
+        }
+    }
+    h.phase = PH_IDLE;
+    return write_scalars(c);
+}
+}  // namespace
+
+int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const double* phi_target, double w_front,
+                  double front_target, double* loss, double* grad) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (c->ndom != 1 || c->mx || c->prm[0].phase_only || steps < 1 || every < 1 || !dts || !phi_target || !loss ||
+        !grad || !std::isfinite(w_front) || w_front < 0)
+        return PEARL_ERR_ARGUMENT;
+    for (int k = 0; k < steps; ++k)
+        if (!(dts[k] > 0) || !std::isfinite(dts[k])) return PEARL_ERR_ARGUMENT;
+    const KP& P = c->kp[0];
+    const size_t n = static_cast<size_t>(P.n);
+    const int segs = (steps + every - 1) / every;
+    const int seg_len = std::min(every, steps);
+    // device memory: checkpoints, the states of one segment, the final state,
+    // two gradient sets, the trial-phase gradient, v and the accumulators
+    const size_t state = 5 * n;
+    const size_t total = state * (static_cast<size_t>(segs) + seg_len + 1 + 1) + 2 * state + 6 * n + ADJ_THETA;
+    double* mem = nullptr;
+    if (hipMalloc(&mem, total * sizeof(double)) != hipSuccess) return fail(c, PEARL_ERR_ALLOC, "adjoint: out of memory");
+    struct Free {
+        double* p;
+        ~Free() { (void)hipFree(p); }
+    } guard{mem};
+    double* ck = mem;
+    double* seg = ck + state * segs;
+    double* fin = seg + state * (seg_len + 1);
+    double* G[2] = {fin + state, fin + 2 * state};
+    double* gphit = G[1] + state;
+    double* vbuf = gphit + 3 * n;
+    double* theta = vbuf + 3 * n;
+    auto ph = [&](double* s5) { return s5; };
+    auto cc = [&](double* s5) { return s5 + 3 * n; };
+    auto mm = [&](double* s5) { return s5 + 4 * n; };
+    st = read_scalars(c);
+    if (st) return st;
+    DevScal& h = c->hS[0];
+    auto save = [&](double* s5) {
+        return copy_state(c, ph(s5), cc(s5), mm(s5), dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur));
+    };
+    auto load = [&](double* s5) {
+        return copy_state(c, dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur), ph(s5), cc(s5), mm(s5));
+    };
+    // forward with checkpoints
+    if ((st = save(ck))) return st;
+    for (int k = 0; k < steps; ++k) {
+        if ((st = step_fixed(c, dts[k]))) return st;
+        if ((k + 1) % every == 0 && (k + 1) < steps && (st = save(ck + state * ((k + 1) / every)))) return st;
+    }
+    if ((st = save(fin))) return st;
+    const double t_end = h.time;
+    const long long step_end = h.step;
+    // loss and its gradient w.r.t. the final state (host)
+    std::vector<double> phiN(3 * n), g(5 * n, 0.0);
+    HIP_CHECK(c, hipMemcpyAsync(phiN.data(), ph(fin), 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    // compensated (Neumaier) sums: the loss and the front height do not
+    // depend on the summation order beyond the last bits
+    struct Sum {
+        double s = 0, c = 0;
+        void add(double v) {
+            const double t = s + v;
+            c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+            s = t;
+        }
+        double value() const { return s + c; }
+    } l2s, hs;
+    for (size_t i = 0; i < 3 * n; ++i) {
+        const double d = phiN[i] - phi_target[i];
+        l2s.add(d * d);
+        g[i] = d;
+    }
+    for (size_t i = 0; i < n; ++i) hs.add(1.0 - phiN[2 * n + i]);
+    const double l2 = l2s.value();
+    const double height = hs.value() * P.dy / P.nx;
+    const double dh_ = height - front_target;
+    *loss = 0.5 * l2 + 0.5 * w_front * dh_ * dh_;
+    for (size_t i = 0; i < n; ++i) g[2 * n + i] -= w_front * dh_ * P.dy / P.nx;
+    HIP_CHECK(c, hipMemcpyAsync(G[1], g.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(theta, 0, ADJ_THETA * sizeof(double), c->stream));
+    // reverse sweep, one checkpointed segment at a time
+    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+    int cur = 1;  // G[cur] holds the gradient w.r.t. the later state
+    for (int sgi = segs - 1; sgi >= 0; --sgi) {
+        const int s0 = sgi * every, len = std::min(every, steps - s0);
+        if ((st = load(ck + state * sgi))) return st;
+        if ((st = save(seg))) return st;
+        for (int k = 0; k < len; ++k) {
+            if ((st = step_fixed(c, dts[s0 + k]))) return st;
+            if ((st = save(seg + state * (k + 1)))) return st;
+        }
+        for (int k = len - 1; k >= 0; --k) {
+            double* s_n = seg + state * k;
+            double* s_n1 = seg + state * (k + 1);
+            double* gin = G[cur];
+            double* gout = G[cur ^ 1];
+            Adj A{ph(s_n), mm(s_n), ph(s_n1), mm(s_n1), ph(gin), cc(gin), mm(gin), ph(gout), cc(gout), mm(gout),
+                  gphit, vbuf, theta, dts[s0 + k]};
+            if ((st = adjoint_solve(c, A))) return st;
+            PEARL_LAUNCH(c, k_adj_diffgrad, grid, c->B, A);
+            hipLaunchKernelGGL(k_adj_diffgrad_fin, one, tb, 0, c->stream, c->B, A);
+            PEARL_LAUNCH(c, k_adj_propose_a, grid, c->B, A);
+            hipLaunchKernelGGL(k_adj_propose_a_fin, one, tb, 0, c->stream, c->B, A);
+            PEARL_LAUNCH(c, k_adj_propose_b, grid, c->B, A);
+            cur ^= 1;
+        }
+    }
+    double th[ADJ_THETA];
+    HIP_CHECK(c, hipMemcpyAsync(th, theta, ADJ_THETA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the context ends on the forward's final state
+    if ((st = load(fin))) return st;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    HIP_CHECK(c, hipGetLastError());
+    for (int q = 0; q < 7; ++q) grad[q] = th[q];
+    st = read_scalars(c);
+    if (st) return st;
+    h.time = t_end;
+    h.step = step_end;
+    h.phase = PH_IDLE;
+    h.e0_valid = 0;
+    h.m0_valid = 0;
+    st = write_scalars(c);
+    if (st) return st;
+    c->sc[0].time = t_end;
+    c->sc[0].step = step_end;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return PEARL_OK;
 }

@@ -609,14 +609,14 @@ __global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
     S->need_state = 0;
 }
 
-// propose_phi :961-1002 for one cell: p and the south/north phases from the
-// column registers, west/east loaded.  v[0] = max change, v[1] = projection
-// failure.
-__device__ __forceinline__ void propose_cell(const KP& P, const double* __restrict__ phi, int n, int i, int y,
-                                             const Col& col, const double (&ps)[3], const double (&p)[3],
-                                             const double (&pn)[3], double m, double dt, double* __restrict__ out,
-                                             double (&v)[2]) {
-    double force[3];
+// Trial value of propose_phi for one cell (:963-996): the force, its mean,
+// the step scale and z = phi - scale*(force - mean), in the reference's
+// operation order.  p and its west, east, south, north neighbours qw, qe,
+// qs, qn (the south/north terms are skipped at the y edges).
+__device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[3], const double (&qw)[3],
+                                          const double (&qe)[3], const double (&qs)[3], const double (&qn)[3],
+                                          double m, double dt, double (&force)[3], double& mean, double& scale,
+                                          double (&z)[3]) {
     const double psi0 = psi(P, 2, m);
     const double dtr[3] = {2.0 * p[1] * p[2], 2.0 * p[0] * p[2], 2.0 * p[0] * p[1]};
     double dpsi[3];
@@ -632,13 +632,10 @@ __device__ __forceinline__ void propose_cell(const KP& P, const double* __restri
             force[a] += div_const(dpsi[b] * dh, P.Vm, P.rVm);
         }
     }
-    double qw[3], qe[3];
-    load_phi(phi, n, i + col.dw, qw);
-    load_phi(phi, n, i + col.de, qe);
 #pragma unroll
     for (int dir = 0; dir < 4; ++dir) {
         if ((dir == 2 && y == 0) || (dir == 3 && y == P.ny - 1)) continue;
-        const double(&q)[3] = dir == 0 ? qw : (dir == 1 ? qe : (dir == 2 ? ps : pn));
+        const double(&q)[3] = dir == 0 ? qw : (dir == 1 ? qe : (dir == 2 ? qs : qn));
         const double dd = dir < 2 ? P.dx2 : P.dy2, rdd = dir < 2 ? P.rdx2 : P.rdy2;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -649,10 +646,24 @@ __device__ __forceinline__ void propose_cell(const KP& P, const double* __restri
                 force[b] -= wv * q[a];
             }
     }
-    const double mean = div_const(force[0] + force[1] + force[2], 3.0, P.r3), scale = dt / (local_tau(P, p) * P.eps);
-    double z[3];
+    mean = div_const(force[0] + force[1] + force[2], 3.0, P.r3);
+    scale = dt / (local_tau(P, p) * P.eps);
 #pragma unroll
     for (int a = 0; a < 3; ++a) z[a] = p[a] - scale * (force[a] - mean);
+}
+
+// propose_phi :961-1002 for one cell: p and the south/north phases from the
+// column registers, west/east loaded.  v[0] = max change, v[1] = projection
+// failure.
+__device__ __forceinline__ void propose_cell(const KP& P, const double* __restrict__ phi, int n, int i, int y,
+                                             const Col& col, const double (&ps)[3], const double (&p)[3],
+                                             const double (&pn)[3], double m, double dt, double* __restrict__ out,
+                                             double (&v)[2]) {
+    double qw[3], qe[3];
+    load_phi(phi, n, i + col.dw, qw);
+    load_phi(phi, n, i + col.de, qe);
+    double force[3], mean, scale, z[3];
+    propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z);
     if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
         v[0] = __builtin_inf();
 #pragma unroll

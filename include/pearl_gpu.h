@@ -165,6 +165,31 @@ int pearl_valid_state(pearl_ctx* ctx, int* valid);
 int pearl_smooth(pearl_ctx* ctx, int max_steps, int min_steps, double energy_rtol, double max_phase_change,
                  pearl_smooth_report* report);
 
+/* One step of size dt that is always taken: propose_phi (:961-1002), then
+ * implicit_diffusion (:1010-1094) on the trial phases, then the trial state
+ * becomes current -- advance() without the acceptance tests and the dt
+ * controller.  The differentiable map of pearl_adjoint; with the same dt
+ * sequence it reproduces an adaptive run's accepted steps.  Coupled mode,
+ * FP64, domain 0.  No reference counterpart. */
+int pearl_step_fixed(pearl_ctx* ctx, double dt);
+
+/* Gradient of the loss
+ *   L = 1/2 sum_cells,phases (phi_N - phi_target)^2 + 1/2 w_front (H - H_target)^2,
+ *   H = dy/nx * sum_cells (1 - phi_gamma)   (the transformed height),
+ * after `steps` fixed steps of sizes dts[0..steps-1] from the current state,
+ * with respect to theta = {sigma, tau_at, tau_ag, tau_tg, D_alpha, D_theta,
+ * D_gamma} (grad[7]).  A hand-written discrete adjoint of the step: the
+ * forward keeps a checkpoint every `every` steps; the reverse sweep
+ * recomputes each segment from its checkpoint and runs the adjoint of the
+ * diffusion solve (one PCG solve with the same operator) and of the
+ * projected phase update.  The diffusion's converged solution is
+ * differentiated as an exact solve (mu' = mu_ref + x), so the gradient is
+ * exact to the PCG tolerance.  On return the context holds the final state.
+ * Coupled mode, FP64, single-domain contexts.  No reference counterpart
+ * (the reference has no derivatives). */
+int pearl_adjoint(pearl_ctx* ctx, int steps, const double* dts, int every, const double* phi_target,
+                  double w_front, double front_target, double* loss, double* grad);
+
 /* Blocked-reduction layout in use: threads per block (256) and cells per
  * thread k.  A sum over an nx-by-ny domain is: tiles of 256 columns by k
  * rows, numbered row of tiles by row of tiles (b = ty*ceil(nx/256) + tx);

@@ -4,6 +4,9 @@ import sys
 
 import numpy as np
 import pytest
+# torch first: libpearl_gpu.so then binds to the HIP runtime torch loads, so
+# tests that use torch on the GPU and the library share one runtime
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG_DIR = os.path.join(ROOT, "fe-c_eutectoid_phase-field_simulation_amd")
