@@ -68,7 +68,7 @@ struct KG {
 
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_adj_prepare(Bufs B, Adj A) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     const int lb = blockIdx.x, n = P.n;
     const double dt = A.dt;
     double* __restrict__ chi = B.chi;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(TPB) void k_adj_prepare(Bufs B, Adj A) {
 }
 
 __global__ __launch_bounds__(TPB) void k_adj_prepare_fin(Bufs B) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     DevScal* S = B.S;
     double v[3];
     if (!final_reduce<3, 0u>(v, B.part + P.poff, P.nb)) return;
@@ -140,7 +140,7 @@ struct XY {
 
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_adj_diffgrad(Bufs B, Adj A) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     const int lb = blockIdx.x, n = P.n;
     const double dt = A.dt;
     const double* __restrict__ lam = B.x;
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(TPB) void k_adj_diffgrad(Bufs B, Adj A) {
 }
 
 __global__ __launch_bounds__(TPB) void k_adj_diffgrad_fin(Bufs B, Adj A) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     double v[3];
     if (!final_reduce<3, 0u>(v, B.part + P.poff, P.nb)) return;
     A.theta[4] += v[0];
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(TPB) void k_adj_diffgrad_fin(Bufs B, Adj A) {
 // ---------------------------------------------------------------------------
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     const int lb = blockIdx.x, n = P.n;
     const double dt = A.dt;
     double th[4] = {0.0, 0.0, 0.0, 0.0};
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
 }
 
 __global__ __launch_bounds__(TPB) void k_adj_propose_a_fin(Bufs B, Adj A) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     double v[4];
     if (!final_reduce<4, 0u>(v, B.part + P.poff, P.nb)) return;
     for (int q = 0; q < 4; ++q) A.theta[q] += v[q];
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a_fin(Bufs B, Adj A) {
 //   g_q[a] -= om * 2 eps sigma p_b/d^2 + v_b w
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_adj_propose_b(Bufs B, Adj A) {
-    const KP& P = B.kp[0];
+    const KP P = B.kp[0];
     const int lb = blockIdx.x, n = P.n;
     const Col col = column<TY>(P, lb, P.ny);
     for_column<TY>(P, col, [&](const int j, const int y) {

@@ -503,7 +503,10 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
     c->total_cells = off;
     c->total_blocks = boff;
     c->total_parts = poff;
-    const size_t plane = static_cast<size_t>(off);
+    // Planes are spaced by the packed cells plus 576 cells (4.5 KiB): with a
+    // power-of-two domain the bare spacing would put the same cell of every
+    // plane on the same cache set and memory channel.
+    const size_t plane = static_cast<size_t>(off) + 576;
     const size_t total = plane * (3 * 2 + 2 + 2 + 11) + static_cast<size_t>(poff) + 64;
     auto bail = [&](int code) {
         pearl_destroy(c);

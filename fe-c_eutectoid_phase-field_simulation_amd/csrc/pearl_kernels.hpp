@@ -413,10 +413,14 @@ struct Where {
 
 // Domain and block-in-domain of this block: a round grid covers all
 // domains; a direct launch (only_d >= 0) covers one domain.
+// The domain index is made wave-uniform (readfirstlane), so the domain's
+// parameters are read through the scalar cache; kernels copy their KP by
+// value, which keeps the fields in registers instead of re-reading them
+// after every global store (the stores could alias the parameter block).
 __device__ __forceinline__ Where where(const Bufs& B, int only_d) {
     const int g = static_cast<int>(blockIdx.x);
     if (only_d >= 0) return {only_d, g};
-    const int d = B.blk2dom[g];
+    const int d = __builtin_amdgcn_readfirstlane(B.blk2dom[g]);
     return {d, g - B.kp[d].boff};
 }
 
@@ -566,7 +570,7 @@ __device__ __forceinline__ void state_cell(const KP& P, const double* phi, const
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_state(Bufs B, int slot) {
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_PROPOSE && S->need_state)) return;
     record(B, slot, w.lb, 1);
@@ -587,7 +591,7 @@ __global__ __launch_bounds__(TPB) void k_state(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_PROPOSE && S->need_state)) return;
     double v[4];
@@ -686,7 +690,7 @@ struct Phases {
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (S->phase != PH_PROPOSE || S->need_state) return;
     record(B, slot, w.lb, 1);
@@ -713,7 +717,7 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (S->phase != PH_PROPOSE || S->need_state) return;
     double v[2];
@@ -770,7 +774,7 @@ template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     using T = Coef<MX>;
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
     record(B, slot, w.lb, 1);
@@ -833,7 +837,7 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_prepare_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
     double v[4];
@@ -872,7 +876,7 @@ template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     using T = Coef<MX>;
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (S->phase != PH_SOLVE) return;
     const int state = S->cg_state;
@@ -887,25 +891,30 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         const T* __restrict__ po = reinterpret_cast<const T*>(pi ? B.p1 : B.p0) + P.off;
         T* __restrict__ pn = reinterpret_cast<T*>(pi ? B.p0 : B.p1) + P.off;
         double* __restrict__ Ap = B.Ap + P.off;
-        const bool fresh = S->restart != 0;
         const double beta = S->beta;
-        // the direction as it is stored; A p is formed from exactly these values
-        auto dir = [&](const int j) {
-            const double zj = z[j];
-            return stored<MX>(fresh ? zj : zj + beta * static_cast<double>(po[j]));
-        };
         double v[1] = {0.0};
-        march<TY, DirK>(
-            P, col, [&](const int j) { return DirK{dir(j), K[j]}; },
-            [&](const int i, const int y, const DirK& s, const DirK& c, const DirK& nn) {
-                const int iw = i + col.dw, ie = i + col.de;
-                double wf[4];
-                face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
-                const double a = apply_vals(chi[i], dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
-                pn[i] = static_cast<T>(c.p);
-                Ap[i] = a;
-                v[0] += c.p * a;
-            });
+        // The restart flag is uniform; one march per value keeps the p_old
+        // loads free of branches (and of the waits a branch forces).
+        auto run = [&](auto fresh) {
+            // the direction as it is stored; A p is formed from exactly these values
+            auto dir = [&](const int j) {
+                const double zj = z[j];
+                return stored<MX>(decltype(fresh)::value ? zj : zj + beta * static_cast<double>(po[j]));
+            };
+            march<TY, DirK>(
+                P, col, [&](const int j) { return DirK{dir(j), K[j]}; },
+                [&](const int i, const int y, const DirK& s, const DirK& c, const DirK& nn) {
+                    const int iw = i + col.dw, ie = i + col.de;
+                    double wf[4];
+                    face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
+                    const double a = apply_vals(chi[i], dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
+                    pn[i] = static_cast<T>(c.p);
+                    Ap[i] = a;
+                    v[0] += c.p * a;
+                });
+        };
+        if (S->restart != 0) run(std::true_type{});
+        else run(std::false_type{});
         block_partial<1, 0u>(v, B.part + P.poff, P.nb, w.lb);
     } else if (state == CG_TRUE) {
         record(B, slot, w.lb, 2);
@@ -933,7 +942,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (S->phase != PH_SOLVE) return;
     const int state = S->cg_state;
@@ -975,7 +984,7 @@ template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     using T = Coef<MX>;
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_ITER && S->pad == 1)) return;
     record(B, slot, w.lb, 1);
@@ -1002,7 +1011,7 @@ __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_cg_b_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_ITER && S->pad == 1)) return;
     double v[2];
@@ -1028,7 +1037,7 @@ struct Fin {
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
     record(B, slot, w.lb, 1);
@@ -1081,7 +1090,7 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
     double v[4];
@@ -1099,7 +1108,7 @@ __global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
     const Where w = where(B, -1);
-    const KP& P = B.kp[w.d];
+    const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
     if (!(S->phase == PH_SOLVE && P.phase_only)) return;
     record(B, slot, w.lb, 1);
@@ -1132,7 +1141,7 @@ __global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
 
 __global__ __launch_bounds__(TPB) void k_phase_only_fin(Bufs B) {
     const int d = blockIdx.x;
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && P.phase_only)) return;
     double v[5];
@@ -1244,7 +1253,7 @@ __global__ void k_clear_kinds(int* kinds, int n) {
 // valid_state, total_carbon and reduced_energy of domain d's current state.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_state_direct(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int cur = S->cur;
@@ -1263,7 +1272,7 @@ __global__ __launch_bounds__(TPB) void k_state_direct(Bufs B, int d) {
 }
 
 __global__ __launch_bounds__(TPB) void k_state_direct_fin(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     double v[4];
     if (!final_reduce<4, (2u | (1u << 6))>(v, B.part + P.poff, P.nb)) return;
@@ -1278,7 +1287,7 @@ __global__ __launch_bounds__(TPB) void k_state_direct_fin(Bufs B, int d) {
 constexpr unsigned DIAG_OPS = 2u | (1u << 2) | (2u << 4) | (1u << 6) | (1u << 8) | (1u << 14);
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int cur = S->cur;
@@ -1307,7 +1316,7 @@ __global__ __launch_bounds__(TPB) void k_diagnostics(Bufs B, int d) {
 }
 
 __global__ __launch_bounds__(TPB) void k_diagnostics_fin(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     double v[8];
     if (!final_reduce<8, DIAG_OPS>(v, B.part + P.poff, P.nb)) return;
@@ -1325,7 +1334,7 @@ __global__ __launch_bounds__(TPB) void k_diagnostics_fin(Bufs B, int d) {
 // crossing height.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_front(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const double* g = B.phi[S->cur] + 3 * P.off + 2 * static_cast<long long>(P.n);
@@ -1341,7 +1350,7 @@ __global__ __launch_bounds__(TPB) void k_front(Bufs B, int d) {
 }
 
 __global__ __launch_bounds__(TPB) void k_front_fin(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     double v[1];
     if (final_reduce<1, 1u>(v, B.part + P.poff, P.nb)) B.S[d].front = v[0];
 }
@@ -1351,7 +1360,7 @@ __global__ __launch_bounds__(TPB) void k_front_fin(Bufs B, int d) {
 // nbs = tiles_x * ceil(rows/TY) blocks.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_strip(Bufs B, int d, int rows, int nbs) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int cur = S->cur;
@@ -1367,7 +1376,7 @@ __global__ __launch_bounds__(TPB) void k_strip(Bufs B, int d, int rows, int nbs)
 }
 
 __global__ __launch_bounds__(TPB) void k_strip_fin(Bufs B, int d, int nbs) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     double v[2];
     if (!final_reduce<2, 1u>(v, B.part + P.poff, nbs)) return;
     B.S[d].gmax = v[0];
@@ -1378,7 +1387,7 @@ __global__ __launch_bounds__(TPB) void k_strip_fin(Bufs B, int d, int nbs) {
 // then close_mu (:789-792) on the new fields.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_shift(Bufs B, int d, int rows, double c_gamma) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int cur = S->cur;
@@ -1451,7 +1460,7 @@ __device__ __forceinline__ void smooth_force_cell(const KP& P, int y, const doub
 
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_smooth_force(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int n = P.n;
@@ -1483,7 +1492,7 @@ __global__ __launch_bounds__(TPB) void k_smooth_force(Bufs B, int d) {
 }
 
 __global__ __launch_bounds__(TPB) void k_smooth_force_fin(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     double v[1];
     if (final_reduce<1, 1u>(v, B.part + P.poff, P.nb)) B.S[d].sm_fmax = v[0];
 }
@@ -1541,7 +1550,7 @@ __device__ __forceinline__ double iface_cell(const KP& P, const double (&p)[3], 
 // east neighbour's is recomputed from phi and the force.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_smooth_trial(Bufs B, int d, double scale) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int n = P.n;
@@ -1567,7 +1576,7 @@ __global__ __launch_bounds__(TPB) void k_smooth_trial(Bufs B, int d, double scal
 }
 
 __global__ __launch_bounds__(TPB) void k_smooth_trial_fin(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     double v[3];
     if (!final_reduce<3, (1u | (1u << 2))>(v, B.part + P.poff, P.nb)) return;
@@ -1579,7 +1588,7 @@ __global__ __launch_bounds__(TPB) void k_smooth_trial_fin(Bufs B, int d) {
 // interface_energy_only of the current phases.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_iface_energy(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     DevScal* S = B.S + d;
     const int lb = blockIdx.x;
     const int n = P.n;
@@ -1602,7 +1611,7 @@ __global__ __launch_bounds__(TPB) void k_iface_energy(Bufs B, int d) {
 }
 
 __global__ __launch_bounds__(TPB) void k_iface_energy_fin(Bufs B, int d) {
-    const KP& P = B.kp[d];
+    const KP P = B.kp[d];
     double v[1];
     if (final_reduce<1, 0u>(v, B.part + P.poff, P.nb)) B.S[d].sm_energy = v[0];
 }
