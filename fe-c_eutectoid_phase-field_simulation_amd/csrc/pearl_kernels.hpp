@@ -504,6 +504,18 @@ __device__ __forceinline__ void march(const KP& P, const Col& c, L&& load, F&& b
     }
 }
 
+// Streaming accesses: a value read once (no stencil reuse) or written for a
+// later kernel moves with the nontemporal hint, so it does not displace the
+// stencil rows that neighbouring columns and tiles re-read from L2.
+template <class T>
+__device__ __forceinline__ T ld_once(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_once(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 // Loop head of implicit_diffusion (:1047-1067) for iteration `it` with the
 // current r.r: info.it, info.residual, then the convergence and
 // iteration-limit branches.  One thread.
@@ -671,7 +683,7 @@ __device__ __forceinline__ void propose_cell(const KP& P, const double* __restri
     if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
         v[0] = __builtin_inf();
 #pragma unroll
-        for (int a = 0; a < 3; ++a) out[a * n + i] = p[a];
+        for (int a = 0; a < 3; ++a) st_once(out + a * n + i, p[a]);
         return;
     }
     double zp[3];
@@ -679,7 +691,7 @@ __device__ __forceinline__ void propose_cell(const KP& P, const double* __restri
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
 #pragma unroll
-    for (int a = 0; a < 3; ++a) out[a * n + i] = zp[a];
+    for (int a = 0; a < 3; ++a) st_once(out + a * n + i, zp[a]);
 }
 
 struct Phases {
@@ -710,7 +722,7 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
             return q;
         },
         [&](const int i, const int y, const Phases& s, const Phases& c, const Phases& nn) {
-            propose_cell(P, phi, n, i, y, col, s.p, c.p, nn.p, mu[i], dt, out, v);
+            propose_cell(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v);
         });
     block_partial<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
@@ -808,13 +820,13 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
         [&](const int i, const int y, const Prep& s, const Prep& c, const Prep& nn) {
             if (!c.ok) v[3] = 1.0;
             const double ic = composition(P.mu_ref, c.m);
-            const double b = c_old[i] - ic;
+            const double b = ld_once(c_old + i) - ic;
             const double ch = c.m.s;
-            chi[i] = ch;
-            icpt[i] = ic;
-            K[i] = c.k;
-            rhs[i] = b;
-            x[i] = c.x;
+            st_once(chi + i, ch);
+            st_once(icpt + i, ic);
+            st_once(K + i, c.k);
+            st_once(rhs + i, b);
+            st_once(x + i, c.x);
             double qw[3], qe[3];
             load_phi(phi_t, n, i + col.dw, qw);
             load_phi(phi_t, n, i + col.de, qe);
@@ -825,9 +837,9 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
             const double d = stored<MX>(ch + dt * (wf[0] + wf[1] + wf[2] + wf[3]));
             const double rr = b - ax;
             const double zz = stored<MX>(rr / d);
-            diag[i] = static_cast<T>(d);
-            r[i] = rr;
-            z[i] = static_cast<T>(zz);
+            st_once(diag + i, static_cast<T>(d));
+            st_once(r + i, rr);
+            st_once(z + i, static_cast<T>(zz));
             v[0] += b * b;
             v[1] += rr * zz;
             v[2] += rr * rr;
@@ -907,9 +919,9 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
                     const int iw = i + col.dw, ie = i + col.de;
                     double wf[4];
                     face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
-                    const double a = apply_vals(chi[i], dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
-                    pn[i] = static_cast<T>(c.p);
-                    Ap[i] = a;
+                    const double a = apply_vals(ld_once(chi + i), dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
+                    st_once(pn + i, static_cast<T>(c.p));
+                    st_once(Ap + i, a);
                     v[0] += c.p * a;
                 });
         };
@@ -929,10 +941,11 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
                 const int iw = i + col.dw, ie = i + col.de;
                 double wf[4];
                 face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
-                const double rr = rhs[i] - apply_vals(chi[i], dt, wf, c.x, x[iw], x[ie], s.x, nn.x);
-                const double zz = stored<MX>(rr / static_cast<double>(diag[i]));
-                r[i] = rr;
-                z[i] = static_cast<T>(zz);
+                const double rr =
+                    ld_once(rhs + i) - apply_vals(ld_once(chi + i), dt, wf, c.x, x[iw], x[ie], s.x, nn.x);
+                const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
+                st_once(r + i, rr);
+                st_once(z + i, static_cast<T>(zz));
                 v[0] += rr * rr;
                 v[1] += rr * zz;
             });
@@ -998,6 +1011,7 @@ __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     double v[2] = {0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     for_column<TY>(P, col, [&](const int i, const int) {
+        // a pure stream: plain accesses measured faster than nontemporal ones here
         x[i] += alpha * static_cast<double>(p[i]);
         const double rr = r[i] - alpha * Ap[i];
         const double zz = stored<MX>(rr / static_cast<double>(diag[i]));
@@ -1071,12 +1085,12 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
             const double xi = c.x;
             const double change =
                 dt * (wf[0] * (x[iw] - xi) + wf[1] * (x[ie] - xi) + wf[2] * (s.x - xi) + wf[3] * (nn.x - xi));
-            const double cn = c_old[i] + change;
-            const double ic = icpt[i], ch = chi[i];
+            const double cn = ld_once(c_old + i) + change;
+            const double ic = ld_once(icpt + i), ch = ld_once(chi + i);
             v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
             const double mn = P.mu_ref + (cn - ic) / ch;
-            c_new[i] = cn;
-            mu_new[i] = mn;
+            st_once(c_new + i, cn);
+            st_once(mu_new + i, mn);
             double qe[3];
             load_phi(phi_t, n, ie, qe);
             double e;
