@@ -6,7 +6,10 @@ phase-diagram input's physics, a colony of 32 ferrite/cementite lamella
 pairs (spacing +/-5 % from a fixed seed, 1:7 cementite:ferrite, colony 256
 rows high) over eutectoid-composition austenite, adaptive dt, implicit
 diffusion, moving window checked at output times.  A "step" is one accepted
-advance() (all its attempts, PCG iterations and acceptance reductions).
+step of the host program's loop (all its attempts, PCG iterations and
+acceptance reductions): pearl_run between output times, with the reference
+loop's output tolerance, then the moving-window check at each output time.
+--host-loop advance times one pearl_advance call per step instead.
 
 Prints one JSON line:
   metric  Mcell-steps/s = nx*ny*accepted steps / wall seconds of the timed loop / 1e6
@@ -407,6 +410,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy", "adjoint"], default="production")
+    ap.add_argument("--host-loop", choices=["run", "advance"], default="run",
+                    help="production: pearl_run between output times (the host program's loop) or one "
+                         "pearl_advance call per step")
     ap.add_argument("--adj-steps", type=int, default=2000, help="adjoint: fixed steps")
     ap.add_argument("--adj-every", type=int, default=100, help="adjoint: checkpoint interval")
     ap.add_argument("--adj-no-reference", action="store_true", help="adjoint: skip the torch autograd reference")
@@ -459,16 +465,32 @@ def main():
     next_output = [(math.floor((0.0 + 1e-12 * cfg.output_dt) / cfg.output_dt) + 1.0) * cfg.output_dt]
     shifts = [0]
 
-    def one_step():
-        s.advance(min(cfg.end_time, next_output[0]))
+    eps_out = 1e-12 * max(cfg.end_time, cfg.output_dt)
+    s.set_stop_tolerance(eps_out)
+
+    def at_output():
         t = s.scalars().time
-        eps = 1e-12 * max(cfg.end_time, cfg.output_dt)
-        if t >= next_output[0] - eps or t >= cfg.end_time - eps:
+        if t >= next_output[0] - eps_out or t >= cfg.end_time - eps_out:
             # the output-time clock snap is applied by the host driver; here
             # only the moving-window check of the hot path is exercised
             sh, _ = s.maybe_shift(cfg.moving_window, cfg.shift_trigger, cfg.shift_target, cfg.shift_gamma_carbon)
             shifts[0] += int(sh)
             next_output[0] += cfg.output_dt
+            return True
+        return False
+
+    def steps(count):
+        """`count` accepted steps of the host program's loop."""
+        left = count
+        while left > 0:
+            if args.host_loop == "advance":
+                s.advance(min(cfg.end_time, next_output[0]))
+                taken = 1
+            else:
+                taken = s.run(min(cfg.end_time, next_output[0]), left)["taken"]
+            left -= taken
+            if not at_output() and taken == 0:
+                raise RuntimeError("the run made no progress")
 
     if args.copy_calibration and have_torch:
         # A plain device-to-device copy of 1 GiB: FETCH_SIZE / WRITE_SIZE of
@@ -479,16 +501,14 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    for _ in range(args.warmup):
-        one_step()
+    steps(args.warmup)
     log(f"warmup {args.warmup} steps done, t = {s.scalars().time:.4e}")
     c0 = s.counters()
     s.profile(not args.no_profile)
     if have_torch:
         torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        one_step()
+    steps(args.steps)
     if have_torch:
         torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -529,7 +549,8 @@ def main():
                                f"lamellar colony ({args.pairs} pairs, +/-5% spacing, "
                                f"seed {args.seed}), {args.undercooling} K undercooling, adaptive dt, implicit "
                                f"diffusion, moving window",
-                   "nx": nx, "ny": ny, "precision": args.precision, "cells_per_thread": rows_per_tile},
+                   "nx": nx, "ny": ny, "precision": args.precision, "cells_per_thread": rows_per_tile,
+                   "host_loop": args.host_loop},
         "attempts": c1["attempts"] - c0["attempts"],
         "rejected_attempts": c1["rejected_attempts"] - c0["rejected_attempts"],
         "cg_iterations": c1["cg_iterations"] - c0["cg_iterations"],

@@ -110,7 +110,7 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
-           "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint"]
+           "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance"]
 
 
 def lib():
@@ -135,6 +135,7 @@ def lib():
         L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
         L.pearl_step_fixed.argtypes = [vp, C.c_double]
+        L.pearl_set_stop_tolerance.argtypes = [vp, C.c_double]
         L.pearl_adjoint.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
         L.pearl_propose_phi.argtypes = [vp, C.c_double, _dp, _dp]
         L.pearl_implicit_diffusion.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.POINTER(DiffusionInfo)]
@@ -268,6 +269,10 @@ class Solver:
         self._check(lib().pearl_run(self._h, stop_time, max_steps, C.byref(taken), C.byref(cg_total),
                                     C.byref(cg_peak), C.byref(last)))
         return dict(taken=taken.value, cg_total=cg_total.value, cg_peak=cg_peak.value, last=last)
+
+    def set_stop_tolerance(self, eps: float):
+        """run()/run_all() end once time >= stop_time - eps (pearl_set_stop_tolerance)."""
+        self._check(lib().pearl_set_stop_tolerance(self._h, float(eps)))
 
     def maybe_shift(self, moving_window: bool, trigger: float, target: float, c_gamma: float):
         sh = C.c_int(0)

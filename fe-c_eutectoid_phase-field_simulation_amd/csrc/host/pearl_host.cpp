@@ -1395,17 +1395,25 @@ int main(int argc, char** argv) {
         snapshot();
         bool stopped = false;
         double saved_time = state.time;
+        // The steps between two outputs run on the device (pearl_run) and end
+        // where the reference's loop takes the output (time >= next_output -
+        // eps, :2307).  A call covers at most kRunChunk steps, so the
+        // interrupt flag is still seen between calls.
+        const double eps = 1e-12 * std::max(config.end_time, config.output_dt);
+        check(gpu, pearl_set_stop_tolerance(gpu, eps));
+        constexpr long long kRunChunk = 512;
         while (sc.time < config.end_time) {
             if (interrupted || (stop_steps > 0 && sc.step - first_step >= stop_steps)) {
                 stopped = true;
                 break;
             }
             if (sc.step >= config.max_steps) throw std::runtime_error("configured max_steps reached before end_time");
-            check(gpu, pearl_advance(gpu, std::min(config.end_time, next_output), &last));
+            long long budget = std::min<long long>(kRunChunk, config.max_steps - sc.step);
+            if (stop_steps > 0) budget = std::min<long long>(budget, stop_steps - (sc.step - first_step));
+            long long taken = 0;
+            check(gpu, pearl_run(gpu, std::min(config.end_time, next_output), budget, &taken, &cg_total, &cg_peak,
+                                 &last));
             check(gpu, pearl_get_state(gpu, nullptr, nullptr, nullptr, &sc));
-            cg_total += last.cg_iterations;
-            cg_peak = std::max(cg_peak, last.cg_iterations);
-            const double eps = 1e-12 * std::max(config.end_time, config.output_dt);
             if (sc.time >= next_output - eps || sc.time >= config.end_time - eps) {
                 // Snap only the clock within rounding tolerance (:2308-2310).
                 bool snapped = false;

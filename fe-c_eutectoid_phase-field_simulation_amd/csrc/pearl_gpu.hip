@@ -57,6 +57,7 @@ struct pearl_ctx {
     std::vector<pearl_params> prm;
     std::vector<KP> kp;             // host copy of the parameter blocks
     std::vector<pearl_scalars> sc;  // host-side copy of the State scalars
+    double stop_eps = 0;            // pearl_set_stop_tolerance
     long long total_cells = 0, total_blocks = 0, total_parts = 0;
     // device memory
     double* mem = nullptr;
@@ -319,6 +320,7 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
         h.taken = 0;
         h.target_steps = max_steps[d];
         h.stop_time = stop_time[d];
+        h.stop_eps = c->stop_eps;
         h.time = c->sc[d].time;
         h.dt_state = c->sc[d].dt;
         h.step = c->sc[d].step;
@@ -333,9 +335,28 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
     }
     st = write_scalars(c);
     if (st) return st;
-    // Rounds per synchronisation: about one accepted step per round; the
-    // batch grows while domains are still running.
-    long long batch = std::min<long long>(std::max<long long>(most, 1), 4);
+    // Rounds per synchronisation.  An accepted step takes at least one round
+    // (more after a rejection or a solve longer than the round's PCG slots),
+    // and a round replayed after every domain has finished costs ~20
+    // predicated launches, more than a synchronisation.  So a batch is the
+    // fewest steps any busy domain can still need: its remaining step
+    // budget, and the steps to its stop time at its current dt (dt only
+    // grows by accepted steps, up to dt_max).  Batches that end short of
+    // the run's end cost one more synchronisation.
+    auto rounds_left = [&]() {
+        long long best = 0;
+        for (int d = 0; d < c->ndom; ++d) {
+            const DevScal& h = c->hS[d];
+            if (h.phase == PH_IDLE) continue;
+            long long need = std::max<long long>(h.target_steps - h.taken, 1);
+            const double span = h.stop_time - h.stop_eps - h.time, dt = std::max(h.dt_state, c->prm[d].dt_max);
+            if (span > 0 && dt > 0 && span / dt < static_cast<double>(need))
+                need = std::max<long long>(1, static_cast<long long>(std::ceil(span / dt)));
+            best = std::max(best, need);
+        }
+        return std::min<long long>(std::max<long long>(best, 1), 64);
+    };
+    long long batch = rounds_left();
     while (true) {
         st = run_rounds(c, static_cast<int>(batch));
         if (st) return st;
@@ -344,7 +365,7 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
         bool busy = false;
         for (int d = 0; d < c->ndom; ++d) busy = busy || c->hS[d].phase != PH_IDLE;
         if (!busy) break;
-        batch = std::min<long long>(batch * 2, 64);
+        batch = rounds_left();
     }
     int peak = 0;
     for (int d = 0; d < c->ndom; ++d) {
@@ -704,6 +725,13 @@ int pearl_run(pearl_ctx* c, double stop_time, long long max_steps, long long* ta
     if (cg_peak) *cg_peak = cgp;
     if (last) *last = info;
     return st;
+}
+
+int pearl_set_stop_tolerance(pearl_ctx* c, double eps) {
+    if (!c) return PEARL_ERR_ARGUMENT;
+    if (!(eps >= 0) || !std::isfinite(eps)) return fail(c, PEARL_ERR_ARGUMENT, "invalid stop tolerance");
+    c->stop_eps = eps;
+    return PEARL_OK;
 }
 
 int pearl_advance(pearl_ctx* c, double stop_time, pearl_step_info* info) {

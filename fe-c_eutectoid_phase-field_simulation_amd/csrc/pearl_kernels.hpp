@@ -83,6 +83,7 @@ struct DevScal {
     int e0_valid, m0_valid;
     long long step, rejected, target_steps, taken;
     double time, dt_state, stop_time;  // State::time, State::dt; stop_time of advance
+    double stop_eps;       // a run ends once time >= stop_time - stop_eps
     double dt;             // dt of the current attempt
     double e0, m0;
     // attempt
@@ -1011,12 +1012,14 @@ __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     double v[2] = {0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     for_column<TY>(P, col, [&](const int i, const int) {
-        // a pure stream: plain accesses measured faster than nontemporal ones here
-        x[i] += alpha * static_cast<double>(p[i]);
-        const double rr = r[i] - alpha * Ap[i];
-        const double zz = stored<MX>(rr / static_cast<double>(diag[i]));
-        r[i] = rr;
-        z[i] = static_cast<T>(zz);
+        // nontemporal throughout: measured against cg_apply run alternately
+        // with it, the pair is 9 % faster than with plain accesses (plain
+        // stores leave dirty lines that the next cg_apply pays for)
+        st_once(x + i, ld_once(x + i) + alpha * static_cast<double>(ld_once(p + i)));
+        const double rr = ld_once(r + i) - alpha * ld_once(Ap + i);
+        const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
+        st_once(r + i, rr);
+        st_once(z + i, static_cast<T>(zz));
         v[0] += rr * zz;
         v[1] += rr * rr;
     });
@@ -1237,7 +1240,7 @@ __global__ void k_decide(Bufs B) {
         if (it > S->cg_peak) S->cg_peak = it;
         S->taken++;
         S->why = -1;
-        S->phase = (S->taken >= S->target_steps || !(S->time < S->stop_time)) ? PH_IDLE : PH_START;
+        S->phase = (S->taken >= S->target_steps || !(S->time < S->stop_time - S->stop_eps)) ? PH_IDLE : PH_START;
         return;
     }
     S->rejected_attempts++;

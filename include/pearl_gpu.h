@@ -133,6 +133,16 @@ int pearl_advance(pearl_ctx* ctx, double stop_time, pearl_step_info* info);
 int pearl_run(pearl_ctx* ctx, double stop_time, long long max_steps, long long* taken,
               long long* cg_total, int* cg_peak, pearl_step_info* last);
 
+/* Output-time tolerance of pearl_run and pearl_batch_run: a domain's run
+ * ends after the first accepted step whose time reaches stop_time - eps.
+ * The reference's loop takes a step to be at the output time when
+ * time >= next_output - eps with eps = 1e-12*max(end_time, output_dt)
+ * (:2307); with that eps, pearl_run(min(end_time, next_output), ...) ends
+ * exactly where the reference's loop writes its next output, so the loop
+ * between two outputs runs without a host round trip per step.  Default 0
+ * (run until time >= stop_time).  Applies to every domain of the context. */
+int pearl_set_stop_tolerance(pearl_ctx* ctx, double eps);
+
 /* Moving window at an output time (replaces maybe_shift :2053-2071 and
  * shift_rows :2023-2052; close_mu :789).  front receives the detected front
  * position (-1 if none). */
