@@ -54,10 +54,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(pkg, nx, ny, pairs, strip_rows, seed, undercooling, tmpdir):
+def build_workload(pkg, nx, ny, pairs, strip_rows, seed, undercooling, tmpdir, overrides=None):
     import importlib
     inputs = importlib.import_module("pearl_amd.inputs")
-    return inputs.colony_workload(nx, ny, pairs, strip_rows, seed, undercooling, tmpdir)
+    return inputs.colony_workload(nx, ny, pairs, strip_rows, seed, undercooling, tmpdir, overrides=overrides)
 
 
 def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
@@ -373,7 +373,7 @@ def run_adjoint(args, pkg):
         t_ref = time.perf_counter() - t0
         dphi = float(np.max(np.abs(prob.final_phases(theta) - phr)))
         ref = {"seconds": round(t_ref, 2), "loss": Lr, "gradient": [float(v) for v in gr],
-               "rel": [float(abs(grad[k] - gr[k]) / abs(gr[k])) for k in range(7)],
+               "rel": [float(abs(grad[k] - gr[k]) / abs(gr[k])) for k in range(len(TA.THETA))],
                "final_phases_max_diff": dphi,
                "what": "the step restated in torch (FP64, stencil PCG solve differentiated implicitly, "
                        f"checkpointed every {every} steps) on the same GPU, autograd"}
@@ -384,8 +384,9 @@ def run_adjoint(args, pkg):
         "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": round(wall * 1e3 / steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"checkpointed adjoint (forward + reverse) of {steps} fixed steps on {nx}x{ny}, "
-                               f"{pairs} lamella pairs, {uc} K, checkpoints every {every}; target from sigma, "
-                               f"tau_ag, tau_tg scaled by seeded factors in [0.8, 1.2]",
+                               f"{pairs} lamella pairs, {uc} K, checkpoints every {every}; target from the "
+                               f"ferrite/austenite and cementite/austenite interface energies and mobility "
+                               f"coefficients scaled by seeded factors in [0.8, 1.2]",
                    "nx": nx, "ny": ny, "every": every},
         "factors": factors, "loss": L.item(), "loss_forward_only": L_fwd,
         "theta": dict(zip(TA.THETA, theta)), "gradient": dict(zip(TA.THETA, [float(v) for v in grad])),
@@ -410,6 +411,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="time without per-kernel events")
     ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy", "adjoint"], default="production")
+    ap.add_argument("--shift-trigger", type=float, default=None,
+                    help="production: moving-window trigger as a fraction of the domain height (default: the "
+                         "input default 0.65, which this colony does not reach)")
+    ap.add_argument("--shift-target", type=float, default=None,
+                    help="production: shift target as a fraction of the height (default: half the trigger)")
+    ap.add_argument("--output-dt", type=float, default=None,
+                    help="production: output interval in seconds (default 3e-10; 1e-10 with --shift-trigger)")
     ap.add_argument("--host-loop", choices=["run", "advance"], default="run",
                     help="production: pearl_run between output times (the host program's loop) or one "
                          "pearl_advance call per step")
@@ -454,7 +462,20 @@ def main():
     nx, ny = args.nx, args.ny
     strip_rows = ny // 8
     tmp = tempfile.mkdtemp()
-    path, cfg, phi, c, mu = build_workload(pkg, nx, ny, args.pairs, strip_rows, args.seed, args.undercooling, tmp)
+    window = None
+    if args.shift_trigger is not None:
+        # a lower trigger than the input default (0.65 of the height), so the
+        # front detection, strip check and row shift run at output times
+        import importlib
+        dy = float(dict(importlib.import_module("pearl_amd.inputs").DEFAULT_PARAMETERS)["dx"])
+        tgt = args.shift_target if args.shift_target is not None else args.shift_trigger / 2
+        window = {"shift_trigger": repr(args.shift_trigger * ny * dy), "shift_target": repr(tgt * ny * dy),
+                  # outputs (and with them the window check) every ~10 steps
+                  "output_dt": repr(args.output_dt if args.output_dt else 1e-10)}
+    elif args.output_dt:
+        window = {"output_dt": repr(args.output_dt)}
+    path, cfg, phi, c, mu = build_workload(pkg, nx, ny, args.pairs, strip_rows, args.seed, args.undercooling, tmp,
+                                           overrides=window)
     log(f"workload {nx}x{ny}: {args.pairs} pairs, colony {strip_rows} rows, {args.undercooling} K")
     precision = pkg.MIXED if args.precision == "mixed" else pkg.FP64
     s = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=precision)
@@ -550,7 +571,8 @@ def main():
                                f"seed {args.seed}), {args.undercooling} K undercooling, adaptive dt, implicit "
                                f"diffusion, moving window",
                    "nx": nx, "ny": ny, "precision": args.precision, "cells_per_thread": rows_per_tile,
-                   "host_loop": args.host_loop},
+                   "host_loop": args.host_loop, "shift_trigger": cfg.shift_trigger,
+                   "shift_target": cfg.shift_target},
         "attempts": c1["attempts"] - c0["attempts"],
         "rejected_attempts": c1["rejected_attempts"] - c0["rejected_attempts"],
         "cg_iterations": c1["cg_iterations"] - c0["cg_iterations"],

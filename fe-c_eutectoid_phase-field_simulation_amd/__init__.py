@@ -56,6 +56,7 @@ class Params(C.Structure):
         ("cg_rtol", C.c_double), ("cg_atol", C.c_double), ("closure_tol", C.c_double),
         ("energy_rtol", C.c_double), ("energy_atol", C.c_double),
         ("cg_max", C.c_int), ("max_retries", C.c_int), ("check_energy", C.c_int), ("phase_only", C.c_int),
+        ("pair_sigma", C.c_double * 3),
     ]
 
 
@@ -165,6 +166,10 @@ PARAM_KEYS = ["A0", "B0", "C0", "D0", "A1", "B1", "C1", "D1", "A2", "B2", "C2", 
               "energy_rtol", "energy_atol", "cg_max", "max_retries", "check_energy", "phase_only"]
 
 
+PAIR_SIGMA_KEYS = ("sigma_at", "sigma_ag", "sigma_tg")
+ADJOINT_GRAD = ("sigma", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2", "sigma_at", "sigma_ag", "sigma_tg")
+
+
 def params_from_dict(d: dict) -> Params:
     """Build pearl_params from a dict with the keys of PARAM_KEYS."""
     p = Params()
@@ -180,6 +185,9 @@ def params_from_dict(d: dict) -> Params:
         setattr(p, k, float(d[k]))
     p.cg_max, p.max_retries = int(d["cg_max"]), int(d["max_retries"])
     p.check_energy, p.phase_only = int(d["check_energy"]), int(d["phase_only"])
+    # optional extension: interface energy per phase pair (0 = sigma)
+    for i, k in enumerate(PAIR_SIGMA_KEYS):
+        p.pair_sigma[i] = float(d.get(k, 0.0))
     return p
 
 
@@ -291,13 +299,14 @@ class Solver:
         self._check(lib().pearl_step_fixed(self._h, float(dt)))
 
     def adjoint(self, dts, phi_target, every: int = 100, w_front: float = 0.0, front_target: float = 0.0):
-        """Loss and gradient w.r.t. (sigma, tau_at, tau_ag, tau_tg, D_alpha,
-        D_theta, D_gamma) after fixed steps dts from the current state; see
-        pearl_adjoint.  The context ends on the final state."""
+        """Loss and gradient (ADJOINT_GRAD order: sigma, tau_at, tau_ag,
+        tau_tg, D_alpha, D_theta, D_gamma, sigma_at, sigma_ag, sigma_tg)
+        after fixed steps dts from the current state; see pearl_adjoint.
+        The context ends on the final state."""
         dts = np.ascontiguousarray(dts, dtype=np.float64)
         tgt = np.ascontiguousarray(phi_target, dtype=np.float64).reshape(3, self.ny, self.nx)
         loss = C.c_double(0)
-        grad = np.zeros(7)
+        grad = np.zeros(len(ADJOINT_GRAD))
         self._check(lib().pearl_adjoint(self._h, int(dts.size), _ptr(dts), int(every), _ptr(tgt), float(w_front),
                                         float(front_target), C.byref(loss), _ptr(grad)))
         return loss.value, grad

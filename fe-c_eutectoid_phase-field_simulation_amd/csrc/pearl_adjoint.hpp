@@ -21,7 +21,7 @@
 //   gz  = J_P^T gphi'  (on the active set: gphi'_a - mean over the set)
 //   u = -gz,  v = s (u - mean u)
 //   gphi = gz + sum_a v_a dF_a/dphi + (u.G) ds/dphi + (neighbour terms)
-//   gmu  = sum_a v_a dF_a/dmu,  gsigma = sum_a v_a F_a^(sigma)/sigma,
+//   gmu  = sum_a v_a dF_a/dmu,  gsigma_ab = sum_a v_a F_a^(sigma_ab)/sigma_ab,
 //   gtau_k = (u.G) ds/dtau_k,   gD_p = sum_i gK_i phi'_p chi_p
 // The capillary force of a cell depends on its neighbours' phases; their
 // gradient terms are gathered in a second pass (each cell collects what its
@@ -32,7 +32,9 @@
 
 namespace pearl_gpu {
 
-constexpr int ADJ_THETA = 8;  // sigma, tau_at, tau_ag, tau_tg, D_alpha, D_theta, D_gamma, (unused)
+// Parameter-gradient accumulators: sigma_at, sigma_ag, sigma_tg (interface
+// energy per phase pair), tau_at, tau_ag, tau_tg, D_alpha, D_theta, D_gamma.
+constexpr int ADJ_THETA = 9;
 
 struct Adj {
     const double* phi0;  // state n (3 planes)
@@ -176,9 +178,9 @@ __global__ __launch_bounds__(TPB) void k_adj_diffgrad_fin(Bufs B, Adj A) {
     const KP P = B.kp[0];
     double v[3];
     if (!final_reduce<3, 0u>(v, B.part + P.poff, P.nb)) return;
-    A.theta[4] += v[0];
-    A.theta[5] += v[1];
-    A.theta[6] += v[2];
+    A.theta[6] += v[0];
+    A.theta[7] += v[1];
+    A.theta[8] += v[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -191,7 +193,8 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
     const KP P = B.kp[0];
     const int lb = blockIdx.x, n = P.n;
     const double dt = A.dt;
-    double th[4] = {0.0, 0.0, 0.0, 0.0};
+    // sigma per pair (01, 02, 12), then tau_at, tau_ag, tau_tg
+    double th[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const Col col = column<TY>(P, lb, P.ny);
     march<TY, Phases>(
         P, col,
@@ -248,10 +251,14 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
                 A.v[a * n + i] = vv[a];
             }
             double g[3] = {gz[0], gz[1], gz[2]};
-            // obstacle obs (p_a1 + p_a2) and triple tr p_a1 p_a2
-            const double V = vv[0] + vv[1] + vv[2];
+            // obstacle sum_{a<b} obs_ab (v_a p_b + v_b p_a) and triple tr p_a1 p_a2
 #pragma unroll
-            for (int c = 0; c < 3; ++c) g[c] += P.obs * (V - vv[c]);
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = a + 1; b < 3; ++b) {
+                    g[a] += P.obs_p[pair_of(a, b)] * vv[b];
+                    g[b] += P.obs_p[pair_of(a, b)] * vv[a];
+                }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
@@ -281,25 +288,29 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
                 for (int b = 0; b < 3; ++b) dF += ddpsi[b] * (dtr + (b == a ? 6.0 * p[b] * (1.0 - p[b]) : 0.0));
                 gmu += vv[a] * dF * P.rVm;
             }
-            // capillary, own-phase part; the sigma-proportional force
-            double fsig[3];
+            // capillary, own-phase part.  Every sigma_ab-proportional force
+            // term, contracted with v, goes to the pair's accumulator
+            // (v . dF/dsigma_ab = that sum / sigma_ab).
+            double vsig[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) fsig[a] = P.obs * (p[(a + 1) % 3] + p[(a + 2) % 3]);
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = a + 1; b < 3; ++b) vsig[pair_of(a, b)] = P.obs_p[pair_of(a, b)] * (vv[a] * p[b] + vv[b] * p[a]);
 #pragma unroll
             for (int dir = 0; dir < 4; ++dir) {
                 if ((dir == 2 && y == 0) || (dir == 3 && y == P.ny - 1)) continue;
                 const double(&q)[3] = dir == 0 ? qw : (dir == 1 ? qe : (dir == 2 ? ps.p : pn.p));
-                const double fc = P.face_coef * (dir < 2 ? P.rdx2 : P.rdy2);
+                const double rd = dir < 2 ? P.rdx2 : P.rdy2;
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
 #pragma unroll
                     for (int b = a + 1; b < 3; ++b) {
+                        const double fc = P.fc_p[pair_of(a, b)] * rd;
                         const double om = vv[a] * q[b] - vv[b] * q[a];
                         g[a] += om * fc * q[b];
                         g[b] -= om * fc * q[a];
                         const double wv = fc * (p[a] * q[b] - q[a] * p[b]);
-                        fsig[a] += wv * q[b];
-                        fsig[b] -= wv * q[a];
+                        vsig[pair_of(a, b)] += wv * om;
                     }
             }
             // the step scale s = dt/(tau_loc eps)
@@ -313,34 +324,35 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
                 const double dS[3] = {p[1] + p[2], p[0] + p[2], p[0] + p[1]};
 #pragma unroll
                 for (int c = 0; c < 3; ++c) g[c] += UG * ds_dtl * (dN[c] - tl * dS[c]) / S;
-                th[1] += UG * ds_dtl * w0 / S;
-                th[2] += UG * ds_dtl * w1 / S;
-                th[3] += UG * ds_dtl * w2 / S;
+                th[3] += UG * ds_dtl * w0 / S;
+                th[4] += UG * ds_dtl * w1 / S;
+                th[5] += UG * ds_dtl * w2 / S;
             } else {
                 const int kmin = (P.tau[0] <= P.tau[1] && P.tau[0] <= P.tau[2]) ? 0 : (P.tau[1] <= P.tau[2] ? 1 : 2);
-                th[1 + kmin] += UG * ds_dtl;
+                th[3 + kmin] += UG * ds_dtl;
             }
-            th[0] += (vv[0] * fsig[0] + vv[1] * fsig[1] + vv[2] * fsig[2]) / P.sigma;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) th[k] += vsig[k] / P.sig_p[k];
 #pragma unroll
             for (int a = 0; a < 3; ++a) A.gphi0[a * n + i] = g[a];
             A.gmu0[i] = gmu;
         });
-    block_partial<4, 0u>(th, B.part + P.poff, P.nb, lb);
+    block_partial<6, 0u>(th, B.part + P.poff, P.nb, lb);
 }
 
 __global__ __launch_bounds__(TPB) void k_adj_propose_a_fin(Bufs B, Adj A) {
     const KP P = B.kp[0];
-    double v[4];
-    if (!final_reduce<4, 0u>(v, B.part + P.poff, P.nb)) return;
-    for (int q = 0; q < 4; ++q) A.theta[q] += v[q];
+    double v[6];
+    if (!final_reduce<6, 0u>(v, B.part + P.poff, P.nb)) return;
+    for (int q = 0; q < 6; ++q) A.theta[q] += v[q];
 }
 
 // Propose adjoint, pass B: each cell gathers what the capillary forces of
 // its neighbours owe it.  Neighbour i with phases p and v(i) coupled to this
 // cell (phases q) over a face of spacing d: for every pair a < b, with
-// w = 2 eps sigma (p_a q_b - q_a p_b)/d^2 and om = v_a q_b - v_b q_a,
-//   g_q[b] += om * 2 eps sigma p_a/d^2 + v_a w
-//   g_q[a] -= om * 2 eps sigma p_b/d^2 + v_b w
+// w = 2 eps sigma_ab (p_a q_b - q_a p_b)/d^2 and om = v_a q_b - v_b q_a,
+//   g_q[b] += om * 2 eps sigma_ab p_a/d^2 + v_a w
+//   g_q[a] -= om * 2 eps sigma_ab p_b/d^2 + v_b w
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_adj_propose_b(Bufs B, Adj A) {
     const KP P = B.kp[0];
@@ -358,11 +370,12 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_b(Bufs B, Adj A) {
             double p[3], vv[3];
             load_phi(A.phi0, n, i, p);
             load_phi(A.v, n, i, vv);
-            const double fc = P.face_coef * (dir < 2 ? P.rdx2 : P.rdy2);
+            const double rd = dir < 2 ? P.rdx2 : P.rdy2;
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = a + 1; b < 3; ++b) {
+                    const double fc = P.fc_p[pair_of(a, b)] * rd;
                     const double w = fc * (p[a] * q[b] - q[a] * p[b]);
                     const double om = vv[a] * q[b] - vv[b] * q[a];
                     g[b] += om * fc * p[a] + vv[a] * w;

@@ -388,6 +388,8 @@ bool params_ok(const pearl_params* p) {
     for (int i = 0; i < 3; ++i)
         if (!(p->A[i] > 0) || !std::isfinite(p->A[i]) || !(p->tau[i] > 0)) return false;
     if (!(p->eps > 0) || !(p->sigma > 0) || !(p->Vm > 0) || p->triple < 0) return false;
+    for (int i = 0; i < 3; ++i)
+        if (!(p->pair_sigma[i] >= 0) || !std::isfinite(p->pair_sigma[i])) return false;
     return true;
 }
 
@@ -429,6 +431,15 @@ KP make_kp(const pearl_params* params, double mu_reference, int k) {
     P.tr = params->triple / eps;
     P.face_coef = 2 * eps * sigma;
     P.eps_sigma = eps * sigma;
+    P.pair = 0;
+    for (int k = 0; k < 3; ++k) {
+        const double s = params->pair_sigma[k] > 0 ? params->pair_sigma[k] : sigma;
+        if (s != sigma) P.pair = 1;
+        P.sig_p[k] = s;
+        P.obs_p[k] = 16 * s / (std::acos(-1.) * std::acos(-1.) * eps);
+        P.fc_p[k] = 2 * eps * s;
+        P.es_p[k] = eps * s;
+    }
     P.eps = eps;
     P.sigma = sigma;
     P.Vm = params->Vm;
@@ -1133,7 +1144,11 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
     if ((st = load(fin))) return st;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     HIP_CHECK(c, hipGetLastError());
-    for (int q = 0; q < 7; ++q) grad[q] = th[q];
+    // grad: sigma (the three pair energies moved together), tau_at, tau_ag,
+    // tau_tg, D_alpha, D_theta, D_gamma, then sigma_at, sigma_ag, sigma_tg
+    grad[0] = th[0] + th[1] + th[2];
+    for (int q = 0; q < 6; ++q) grad[1 + q] = th[3 + q];
+    for (int q = 0; q < 3; ++q) grad[7 + q] = th[q];
     st = read_scalars(c);
     if (st) return st;
     h.time = t_end;

@@ -49,6 +49,10 @@ struct KP {
     double obs, tr;         // 16*sigma/(pi*pi*eps), triple/eps   :964
     double face_coef;       // 2*eps*sigma                        :981
     double eps_sigma;       // eps*sigma                          :816
+    // per phase pair (a<b: 01, 02, 12): 16 sigma_ab/(pi^2 eps), 2 eps sigma_ab,
+    // eps sigma_ab; all equal to obs / face_coef / eps_sigma unless `pair`
+    int pair;               // pairwise interface energies in use
+    double sig_p[3], obs_p[3], fc_p[3], es_p[3];
     double eps, sigma, Vm;
     double tau[3], tau_min;
     double psiref[3];       // psi_a(mu_ref) - psi_gamma(mu_ref), psiref[2]=0  :797
@@ -225,6 +229,25 @@ __device__ __forceinline__ bool final_reduce(double (&v)[NQ], const double* __re
 // ---------------------------------------------------------------------------
 // Pointwise physics (pearl_serial.cpp:76-145, :533-537, :491-509)
 // ---------------------------------------------------------------------------
+// Index of the phase pair a < b: 01 -> 0, 02 -> 1, 12 -> 2.
+__device__ __forceinline__ constexpr int pair_of(int a, int b) { return a + b - 1; }
+
+// Obstacle force on phase a: the reference's obs (phi_a1 + phi_a2) (:964),
+// or with pairwise interface energies obs_{a,a1} phi_a1 + obs_{a,a2} phi_a2.
+__device__ __forceinline__ double obstacle_force(const KP& P, const double (&p)[3], int a) {
+    const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+    if (P.pair) return P.obs_p[pair_of(a < a1 ? a : a1, a < a1 ? a1 : a)] * p[a1] +
+                       P.obs_p[pair_of(a < a2 ? a : a2, a < a2 ? a2 : a)] * p[a2];
+    return P.obs * (p[a1] + p[a2]);
+}
+
+// Obstacle energy density: obs sum_{a<b} phi_a phi_b (:802), or with
+// pairwise interface energies sum_{a<b} obs_ab phi_a phi_b.
+__device__ __forceinline__ double obstacle_energy(const KP& P, const double (&p)[3]) {
+    if (P.pair) return P.obs_p[0] * (p[0] * p[1]) + P.obs_p[1] * (p[0] * p[2]) + P.obs_p[2] * (p[1] * p[2]);
+    return P.obs * (p[0] * p[1] + p[0] * p[2] + p[1] * p[2]);
+}
+
 __device__ __forceinline__ double psi(const KP& P, int p, double mu) {  // :85-89
     const double d = mu - P.B[p];
     return __builtin_fma(P.qpsi[p], d * d, P.C[p]);
@@ -361,13 +384,13 @@ __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], c
         const double dc = c - composition(P.mu_ref, m);
         chem += .5 * dc * dc / m.s;
     }
-    double v = div_const(chem, P.Vm, P.rVm) + P.obs * (p[0] * p[1] + p[0] * p[2] + p[1] * p[2]) + P.tr * p[0] * p[1] * p[2];
+    double v = div_const(chem, P.Vm, P.rVm) + obstacle_energy(P, p) + P.tr * p[0] * p[1] * p[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int bb = a + 1; bb < 3; ++bb) {
             const double z = div_const(p[a] * qe[bb] - qe[a] * p[bb], P.dx, P.rdx);
-            v += P.eps_sigma * z * z;
+            v += P.es_p[pair_of(a, bb)] * z * z;
         }
     if (north) {
 #pragma unroll
@@ -375,7 +398,7 @@ __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], c
 #pragma unroll
             for (int bb = a + 1; bb < 3; ++bb) {
                 const double z = div_const(p[a] * qn[bb] - qn[a] * p[bb], P.dy, P.rdy);
-                v += P.eps_sigma * z * z;
+                v += P.es_p[pair_of(a, bb)] * z * z;
             }
     }
     energy = v;
@@ -642,7 +665,7 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
-        force[a] = P.obs * (p[a1] + p[a2]) + P.tr * p[a1] * p[a2];
+        force[a] = obstacle_force(P, p, a) + P.tr * p[a1] * p[a2];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const double dh = dtr[a] + (b == a ? 6.0 * p[b] * (1.0 - p[b]) : 0.0);
@@ -658,7 +681,7 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = a + 1; b < 3; ++b) {
-                const double wv = div_const(P.face_coef * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                const double wv = div_const(P.fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
                 force[a] += wv * q[b];
                 force[b] -= wv * q[a];
             }
@@ -1454,7 +1477,7 @@ __device__ __forceinline__ void smooth_force_cell(const KP& P, int y, const doub
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
-        f[a] = P.obs * (p[a1] + p[a2]) + P.tr * p[a1] * p[a2];
+        f[a] = obstacle_force(P, p, a) + P.tr * p[a1] * p[a2];
     }
 #pragma unroll
     for (int dir = 0; dir < 4; ++dir) {
@@ -1465,7 +1488,7 @@ __device__ __forceinline__ void smooth_force_cell(const KP& P, int y, const doub
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = a + 1; b < 3; ++b) {
-                const double wv = div_const(P.face_coef * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                const double wv = div_const(P.fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
                 f[a] += wv * q[b];
                 f[b] -= wv * q[a];
             }
@@ -1541,13 +1564,13 @@ __device__ __forceinline__ Trial smooth_trial_at(const KP& P, const double* __re
 // and (below the top row) north face terms.
 __device__ __forceinline__ double iface_cell(const KP& P, const double (&p)[3], const double (&qe)[3],
                                              const double (&qn)[3], bool north) {
-    double v = P.obs * (p[0] * p[1] + p[0] * p[2] + p[1] * p[2]) + P.tr * p[0] * p[1] * p[2];
+    double v = obstacle_energy(P, p) + P.tr * p[0] * p[1] * p[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = a + 1; b < 3; ++b) {
             const double z = div_const(p[a] * qe[b] - qe[a] * p[b], P.dx, P.rdx);
-            v += P.eps_sigma * z * z;
+            v += P.es_p[pair_of(a, b)] * z * z;
         }
     if (north) {
 #pragma unroll
@@ -1555,7 +1578,7 @@ __device__ __forceinline__ double iface_cell(const KP& P, const double (&p)[3], 
 #pragma unroll
             for (int b = a + 1; b < 3; ++b) {
                 const double z = div_const(p[a] * qn[b] - qn[a] * p[b], P.dy, P.rdy);
-                v += P.eps_sigma * z * z;
+                v += P.es_p[pair_of(a, b)] * z * z;
             }
     }
     return v;

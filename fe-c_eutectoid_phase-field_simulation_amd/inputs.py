@@ -127,12 +127,13 @@ def lever_rule_fraction(c_mean: float, c_alpha: float, c_theta: float) -> float:
 
 
 def colony_workload(nx: int, ny: int, pairs: int, strip_rows: int, seed: int, undercooling: float, directory: str,
-                    moving_window: int = 1):
+                    moving_window: int = 1, overrides: dict | None = None):
     """The lamellar-colony workload: writes its input file into `directory`
     and returns (path, config, phi[3,ny,nx], carbon, mu).  Pair spacings vary
     by +/-5 % (seeded), cementite:ferrite 1:7, colony rows [0, strip_rows);
     the phases are diffuse (initial.diffuse_colony), the carbon is the one of
-    the common initial chemical potential and mu is uniform."""
+    the common initial chemical potential and mu is uniform.  `overrides`
+    adds input keys (e.g. shift_trigger, shift_target)."""
     import os
 
     import numpy as np
@@ -140,10 +141,11 @@ def colony_workload(nx: int, ny: int, pairs: int, strip_rows: int, seed: int, un
     from . import config, initial
     path = os.path.join(directory, "workload.in")
     fills = lamella_fills(nx, pairs, strip_rows, spacing_noise=0.05, seed=seed)
-    write_input(path, nx, ny, 1e-6, 3e-10, fills,
-                overrides={"undercooling": repr(float(undercooling)),
-                           "max_undercooling": repr(float(max(10.0, math.ceil(undercooling)))),
-                           "moving_window": int(moving_window)})
+    keys = {"undercooling": repr(float(undercooling)),
+            "max_undercooling": repr(float(max(10.0, math.ceil(undercooling)))),
+            "moving_window": int(moving_window)}
+    keys.update(overrides or {})
+    write_input(path, nx, ny, 1e-6, 3e-10, fills, overrides=keys)
     cfg = config.read_config(path)
     phi = initial.diffuse_colony(nx, ny, cfg.dx, cfg.dy, cfg.eps, initial.plates_from_fills(fills), strip_rows)
     c = initial.carbon_at_mu(phi, cfg.thermo, cfg.initial_mu)

@@ -10,9 +10,12 @@ term:
     L = 1/2 sum (phi - phi_target)^2 + 1/2 w_front (H - H_target)^2,
     H = dy / nx * sum over cells of (1 - phi_gamma)   (mean colony height)
 
-Parameters theta = (sigma, tau_at, tau_ag, tau_tg, D_alpha, D_theta,
-D_gamma): the interface energy, the three pair mobilities (the reference's
-tau coefficients) and the three carbon diffusivities.
+Parameters theta = (sigma_at, sigma_ag, sigma_tg, tau_at, tau_ag, tau_tg,
+D_alpha, D_theta, D_gamma): the interface energy of each phase pair
+(ferrite-cementite, ferrite-austenite, cementite-austenite; the reference has
+one sigma for all three, and equal pair energies reproduce it), the three
+pair mobility coefficients (the reference's tau) and the three carbon
+diffusivities.
 
 `FixedStepProblem` holds everything but theta.  `PhaseFieldLoss.apply(theta,
 problem)` runs pearl_adjoint once: the forward with checkpoints every
@@ -33,9 +36,10 @@ import math
 import numpy as np
 import torch
 
-from . import Solver
+from . import ADJOINT_GRAD, Solver
 
-THETA = ("sigma", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2")
+THETA = ("sigma_at", "sigma_ag", "sigma_tg", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2")
+_FROM_ADJOINT = [ADJOINT_GRAD.index(k) for k in THETA]
 
 
 def front_height(phi: np.ndarray, dy: float) -> float:
@@ -79,8 +83,13 @@ class FixedStepProblem:
         p.update(zip(THETA, th))
         return p
 
+    def theta_values(self) -> list:
+        """THETA at the problem's parameters (a pair energy left 0 is sigma)."""
+        return [float(self.params.get(k) or self.params["sigma"]) if k.startswith("sigma_") else float(self.params[k])
+                for k in THETA]
+
     def theta0(self, dtype=torch.float64, device="cpu") -> torch.Tensor:
-        return torch.tensor([self.params[k] for k in THETA], dtype=dtype, device=device)
+        return torch.tensor(self.theta_values(), dtype=dtype, device=device)
 
     def _solver(self, theta):
         s = Solver(self.params_at(theta), self.mu_reference, device=self.device)
@@ -103,10 +112,11 @@ class FixedStepProblem:
         return self.loss_of_phases(self.final_phases(theta))
 
     def loss_and_grad(self, theta):
-        """(loss, dL/dtheta[7]) from one pearl_adjoint call."""
+        """(loss, dL/dtheta in THETA order) from one pearl_adjoint call."""
         with self._solver(theta) as s:
-            return s.adjoint(self.dts, self.phi_target, every=self.every, w_front=self.w_front,
-                             front_target=self.front_target)
+            loss, g = s.adjoint(self.dts, self.phi_target, every=self.every, w_front=self.w_front,
+                                front_target=self.front_target)
+        return loss, g[_FROM_ADJOINT]
 
 
 class PhaseFieldLoss(torch.autograd.Function):
@@ -138,10 +148,10 @@ def colony_inverse_problem(nx=512, ny=256, pairs=4, undercooling=15.0, steps=200
 
     1. An adaptive run (advance) of `steps` accepted steps with the
        unscaled parameters fixes the dt sequence.
-    2. The target: the same fixed steps with sigma, tau_ag (ferrite/austenite)
-       and tau_tg (cementite/austenite) scaled by seeded factors in
-       [0.8, 1.2] (one interface energy in this model, so sigma carries both
-       interfaces' energy).
+    2. The target: the same fixed steps with the ferrite/austenite and
+       cementite/austenite interface energies (sigma_ag, sigma_tg) and
+       mobility coefficients (tau_ag, tau_tg) scaled by seeded factors in
+       [0.8, 1.2].
     3. The loss starts from the unscaled parameters; w_front = nx / dy^2, so
        that a one-cell shift of the front weighs about like the phase
        mismatch of that shift.
@@ -163,13 +173,13 @@ def colony_inverse_problem(nx=512, ny=256, pairs=4, undercooling=15.0, steps=200
         s.set_state(phi, c, mu, scalars)
         dts = np.array([s.advance(1.0).accepted_dt for _ in range(steps)])
     rng = random.Random(factor_seed)
-    factors = {k: rng.uniform(0.8, 1.2) for k in ("sigma", "tau_ag", "tau_tg")}
+    factors = {k: rng.uniform(0.8, 1.2) for k in ("sigma_ag", "sigma_tg", "tau_ag", "tau_tg")}
     scaled = dict(params)
     for k, f in factors.items():
-        scaled[k] = params[k] * f
+        scaled[k] = (params.get(k) or params["sigma"]) * f
     state = (phi, c, mu, scalars)
     pre = FixedStepProblem(scaled, cfg.initial_mu, state, dts, phi, every=every, device=device)
-    phi_target = pre.final_phases([scaled[k] for k in THETA])
+    phi_target = pre.final_phases(pre.theta_values())
     problem = FixedStepProblem(params, cfg.initial_mu, state, dts, phi_target, w_front=nx / cfg.dy ** 2,
                                every=every, device=device)
     return problem, factors, cfg

@@ -71,6 +71,14 @@ typedef struct {
     double cg_rtol, cg_atol, closure_tol;
     double energy_rtol, energy_atol;
     int cg_max, max_retries, check_energy, phase_only;
+    /* Extension (no reference counterpart): interface energy per phase pair
+     * (alpha-theta, alpha-gamma, theta-gamma); 0 means `sigma`.  With every
+     * entry 0 or equal to sigma the step is the reference's, bit for bit.
+     * Otherwise the obstacle term is sum_{a<b} 16 sigma_ab/(pi^2 eps)
+     * phi_a phi_b and the gradient term sum_{a<b} eps sigma_ab q_ab^2 (the
+     * reference's forms with sigma per pair); the energy-test scale keeps
+     * `sigma`.  A zero-initialised struct selects the reference. */
+    double pair_sigma[3];
 } pearl_params;
 
 /* State scalars (State, :758-764). */
@@ -188,7 +196,8 @@ int pearl_step_fixed(pearl_ctx* ctx, double dt);
  *   H = dy/nx * sum_cells (1 - phi_gamma)   (the transformed height),
  * after `steps` fixed steps of sizes dts[0..steps-1] from the current state,
  * with respect to theta = {sigma, tau_at, tau_ag, tau_tg, D_alpha, D_theta,
- * D_gamma} (grad[7]).  A hand-written discrete adjoint of the step: the
+ * D_gamma, sigma_at, sigma_ag, sigma_tg} (grad[10]; grad[0] moves the three
+ * pair energies together and is the sum of grad[7..9]).  A hand-written discrete adjoint of the step: the
  * forward keeps a checkpoint every `every` steps; the reverse sweep
  * recomputes each segment from its checkpoint and runs the adjoint of the
  * diffusion solve (one PCG solve with the same operator) and of the

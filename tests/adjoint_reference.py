@@ -24,7 +24,15 @@ import math
 import torch
 from torch.utils.checkpoint import checkpoint
 
-THETA = ["sigma", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2"]
+# interface energy per phase pair (alpha-theta, alpha-gamma, theta-gamma),
+# mobility coefficients, diffusivities.  The obstacle and gradient terms use
+# sigma per pair; with equal pair energies they are the reference's terms.
+THETA = ["sigma_at", "sigma_ag", "sigma_tg", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2"]
+
+
+def theta_of(pd):
+    """THETA values of a parameter dict (a missing pair energy is sigma)."""
+    return [float(pd.get(k) or pd["sigma"]) if k.startswith("sigma_") else float(pd[k]) for k in THETA]
 
 
 def _shift(f, dy, dx):
@@ -159,7 +167,8 @@ def _dense_solve(chi, K, b, dt, dx2, dy2):
 
 
 def step(phi, c, mu, theta, pd, mu_ref, dt, solve="dense"):
-    sigma, tau_at, tau_ag, tau_tg, D0, D1, D2 = theta
+    s_at, s_ag, s_tg, tau_at, tau_ag, tau_tg, D0, D1, D2 = theta
+    sig = {(0, 1): s_at, (0, 2): s_ag, (1, 2): s_tg}
     eps, Vm, triple = pd["eps"], pd["Vm"], pd["triple"]
     A = [pd["A0"], pd["A1"], pd["A2"]]
     Bq = [pd["B0"], pd["B1"], pd["B2"]]
@@ -167,9 +176,9 @@ def step(phi, c, mu, theta, pd, mu_ref, dt, solve="dense"):
     D = [D0, D1, D2]
     dx2, dy2 = pd["dx"] ** 2, pd["dy"] ** 2
     ny, nx = c.shape
-    obs = 16 * sigma / (math.pi * math.pi * eps)
+    obs = {ab: 16 * v / (math.pi * math.pi * eps) for ab, v in sig.items()}
     tr = triple / eps
-    fc = 2 * eps * sigma
+    fc = {ab: 2 * eps * v for ab, v in sig.items()}
     p = phi
     psi = [Cq[b] - 0.25 / A[b] * (mu - Bq[b]) ** 2 for b in range(3)]
     dpsi = [psi[b] - psi[2] for b in range(3)]
@@ -177,7 +186,7 @@ def step(phi, c, mu, theta, pd, mu_ref, dt, solve="dense"):
     for a in range(3):
         a1, a2 = (a + 1) % 3, (a + 2) % 3
         dtr = 2.0 * p[a1] * p[a2]
-        f = obs * (p[a1] + p[a2]) + tr * p[a1] * p[a2]
+        f = obs[tuple(sorted((a, a1)))] * p[a1] + obs[tuple(sorted((a, a2)))] * p[a2] + tr * p[a1] * p[a2]
         for b in range(3):
             dh = dtr + (6.0 * p[b] * (1.0 - p[b]) if b == a else 0.0)
             f = f + dpsi[b] * dh / Vm
@@ -186,7 +195,7 @@ def step(phi, c, mu, theta, pd, mu_ref, dt, solve="dense"):
         q = _shift(phi, sy, sx)
         for a in range(3):
             for b in range(a + 1, 3):
-                w = fc * (p[a] * q[b] - q[a] * p[b]) / dd
+                w = fc[(a, b)] * (p[a] * q[b] - q[a] * p[b]) / dd
                 if mask is not None:
                     w = torch.where(mask, w, torch.zeros_like(w))
                 F[a] = F[a] + w * q[b]
@@ -223,7 +232,7 @@ def loss_and_grad(pd, mu_ref, phi0, c0, mu0, dts, phi_target, w_front=0.0, front
     every > 0 runs the steps in checkpointed segments of that many steps
     (the graph of one segment is rebuilt during the backward pass)."""
     dev = torch.device(device)
-    th = torch.tensor([pd[k] for k in THETA], dtype=torch.float64, device=dev, requires_grad=True)
+    th = torch.tensor(theta_of(pd), dtype=torch.float64, device=dev, requires_grad=True)
     phi = torch.tensor(phi0, dtype=torch.float64, device=dev)
     c = torch.tensor(c0, dtype=torch.float64, device=dev)
     mu = torch.tensor(mu0, dtype=torch.float64, device=dev)

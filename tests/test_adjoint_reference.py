@@ -10,7 +10,7 @@
 2. The stencil-PCG solve with implicit differentiation (solve="cg") and
    checkpointed segments give the dense-solve gradient.
 3. Autograd of the reference agrees with central differences of the same
-   torch model (24 steps; h = 1e-6 relative for sigma and a mobility,
+   torch model (24 steps; h = 1e-6 relative for two pair energies and a mobility,
    1e-3 for the diffusivities, whose influence on the loss over 24 steps
    is about 1e-3 of sigma's), so the reference gradient is the derivative
    of the model it restates, projection included.
@@ -42,7 +42,7 @@ def test_reference_step_equals_oracle_step(O, solve):
     assert st == 0
     d = O.implicit_diffusion(P, phi_t, g["c0"], g["mu0"], dt, mu_ref)
     assert d["ok"]
-    th = [torch.tensor(pd[k], dtype=torch.float64) for k in R.THETA]
+    th = [torch.tensor(v, dtype=torch.float64) for v in R.theta_of(pd)]
     pt, cn, mn = R.step(torch.tensor(g["phi0"]), torch.tensor(g["c0"]), torch.tensor(g["mu0"]), th, pd, mu_ref, dt,
                         solve)
     assert np.max(np.abs(pt.numpy() - phi_t)) <= 1e-15
@@ -66,11 +66,12 @@ def test_reference_gradient_matches_its_finite_differences():
         return R.loss_and_grad(q, mu_ref, g["phi0"], g["c0"], g["mu0"], dts, g["phi0"], w, ht)[0]
 
     _, gr, *_ = R.loss_and_grad(pd, mu_ref, g["phi0"], g["c0"], g["mu0"], dts, g["phi0"], w, ht)
-    for name, rh in (("sigma", 1e-6), ("tau_tg", 1e-6), ("D0", 1e-3), ("D1", 1e-3)):
+    for name, rh in (("sigma_ag", 1e-6), ("sigma_tg", 1e-6), ("tau_tg", 1e-6), ("D0", 1e-3), ("D1", 1e-3)):
         k = R.THETA.index(name)
-        h = rh * pd[name]
+        base = R.theta_of(pd)[k]
+        h = rh * base
         a, b = dict(pd), dict(pd)
-        a[name] += h
-        b[name] -= h
+        a[name] = base + h
+        b[name] = base - h
         fd = (loss(a) - loss(b)) / (2 * h)
         assert abs(fd - gr[k]) <= 1e-6 * abs(gr[k]), (name, fd, gr[k])

@@ -13,6 +13,13 @@ checkpointed every 50.  The reference is the same restatement with the
 stencil PCG solve, differentiated implicitly, run in checkpointed segments
 on the GPU in FP64.
 
+Parameters: the interface energy of each phase pair (sigma_at, sigma_ag,
+sigma_tg), the mobility coefficients and the diffusivities.  The small case
+runs twice: with the reference's single sigma (pair energies equal; the
+forward is the reference's step) and with three different pair energies.
+The single-sigma gradient (pearl_adjoint's grad[0]) is the sum of the pair
+gradients and is checked against a central difference of `sigma` itself.
+
 Bars: 1e-7 relative against the dense reference, 1e-8 against the colony
 reference; 1e-6 relative against central differences of the GPU forward.
 The difference step is 1e-6 relative for sigma and the mobilities.  The
@@ -30,8 +37,8 @@ import pytest
 from conftest import load_golden
 
 pytestmark = pytest.mark.gpu
-THETA = ["sigma", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2"]
 STEPS = 24
+PAIRS = {"sigma_at": 0.55, "sigma_ag": 0.42, "sigma_tg": 0.61}
 
 
 def say(*a):
@@ -39,9 +46,10 @@ def say(*a):
     sys.stdout.flush()
 
 
-def setup():
+def setup(pairs=None):
     g = load_golden("small_48x40")
     pd = dict(g["pd"])
+    pd.update(pairs or {})
     pd["cg_rtol"], pd["cg_atol"] = 1e-15, 1e-30
     mu_ref = float(g["sc0"][9])
     dts = np.full(STEPS, float(g["dt_probe"]))
@@ -65,6 +73,10 @@ def gpu_adjoint(pkg, g, pd, mu_ref, dts, target, w, h_target, every):
         return s.adjoint(dts, target, every=every, w_front=w, front_target=h_target)
 
 
+def by_name(pkg, grad):
+    return dict(zip(pkg.ADJOINT_GRAD, grad))
+
+
 @pytest.fixture(scope="module")
 def adj(pkg):
     g, pd, mu_ref, dts, target, w, ht = setup()
@@ -72,28 +84,34 @@ def adj(pkg):
     return gpu_adjoint(pkg, g, pd, mu_ref, dts, target, w, ht, 4)
 
 
-def test_adjoint_matches_autograd_reference(pkg, adj):
+@pytest.mark.parametrize("pairs", [None, PAIRS], ids=["single-sigma", "pair-sigma"])
+def test_adjoint_matches_autograd_reference(pkg, pairs):
     import adjoint_reference as R
-    g, pd, mu_ref, dts, target, w, ht = setup()
-    L, grad = adj
+    g, pd, mu_ref, dts, target, w, ht = setup(pairs)
+    say("adjoint vs autograd", "pairs" if pairs else "single sigma")
+    L, grad = gpu_adjoint(pkg, g, pd, mu_ref, dts, target, w, ht, 4)
+    gd = by_name(pkg, grad)
     Lr, gr, *_ = R.loss_and_grad(pd, mu_ref, g["phi0"], g["c0"], g["mu0"], dts, target, w, ht)
     say("loss", L, Lr)
-    for k, name in enumerate(THETA):
-        say(f"  {name:7s} adjoint {grad[k]: .12e} autograd {gr[k]: .12e} rel {abs(grad[k] - gr[k]) / abs(gr[k]):.2e}")
+    for k, name in enumerate(R.THETA):
+        say(f"  {name:8s} adjoint {gd[name]: .12e} autograd {gr[k]: .12e} rel {abs(gd[name] - gr[k]) / abs(gr[k]):.2e}")
     assert abs(L - Lr) <= 1e-9 * abs(Lr)
-    for k in range(7):
-        assert abs(grad[k] - gr[k]) <= 1e-7 * abs(gr[k])
+    for k, name in enumerate(R.THETA):
+        assert abs(gd[name] - gr[k]) <= 1e-7 * abs(gr[k])
+    assert abs(gd["sigma"] - (gd["sigma_at"] + gd["sigma_ag"] + gd["sigma_tg"])) <= 1e-14 * abs(gd["sigma"])
 
 
-H_REL = {"sigma": 1e-6, "tau_at": 1e-6, "tau_ag": 1e-6, "tau_tg": 1e-6, "D0": 1e-3, "D1": 1e-3, "D2": 1e-3}
+H_REL = {"sigma": 1e-6, "sigma_at": 1e-6, "sigma_ag": 1e-6, "sigma_tg": 1e-6, "tau_at": 1e-6, "tau_ag": 1e-6,
+         "tau_tg": 1e-6, "D0": 1e-3, "D1": 1e-3, "D2": 1e-3}
 
 
 def central_difference(pkg, g, pd, mu_ref, dts, target, w, ht, name, rel_h):
-    h = rel_h * pd[name]
+    base = float(pd.get(name) or pd["sigma"])  # a pair energy left unset is sigma
+    h = rel_h * base
     vals = []
     for sgn in (1, -1):
         q = dict(pd)
-        q[name] = pd[name] + sgn * h
+        q[name] = base + sgn * h
         vals.append(gpu_loss(pkg, g, q, mu_ref, dts, target, w, ht))
     return (vals[0] - vals[1]) / (2 * h)
 
@@ -101,14 +119,16 @@ def central_difference(pkg, g, pd, mu_ref, dts, target, w, ht, name, rel_h):
 def test_adjoint_matches_finite_differences(pkg, adj):
     g, pd, mu_ref, dts, target, w, ht = setup()
     L, grad = adj
-    for k, name in enumerate(THETA):
+    gd = by_name(pkg, grad)
+    for name in pkg.ADJOINT_GRAD:
+        k = pkg.ADJOINT_GRAD.index(name)
         fd = central_difference(pkg, g, pd, mu_ref, dts, target, w, ht, name, H_REL[name])
         rel = abs(grad[k] - fd) / abs(fd)
         extra = ""
         if H_REL[name] != 1e-6:
             fd6 = central_difference(pkg, g, pd, mu_ref, dts, target, w, ht, name, 1e-6)
             extra = f"   (h=1e-6: fd {fd6: .12e} rel {abs(grad[k] - fd6) / abs(fd6):.2e})"
-        say(f"  {name:7s} adjoint {grad[k]: .12e} fd(h={H_REL[name]:.0e}) {fd: .12e} rel {rel:.2e}{extra}")
+        say(f"  {name:8s} adjoint {grad[k]: .12e} fd(h={H_REL[name]:.0e}) {fd: .12e} rel {rel:.2e}{extra}")
         assert rel <= 1e-6
 
 
@@ -149,7 +169,7 @@ def test_colony_adjoint_matches_reference(pkg, colony):
         say(f"  {name:7s} adjoint {g[k]: .12e} reference {gr[k]: .12e} rel {abs(g[k] - gr[k]) / abs(gr[k]):.2e}")
     assert abs(L.item() - Lr) <= 1e-9 * abs(Lr)
     assert float(np.max(np.abs(prob.final_phases(prob.theta0().tolist()) - phr))) <= 1e-11
-    for k in range(7):
+    for k in range(len(TA.THETA)):
         assert abs(g[k] - gr[k]) <= 1e-8 * abs(gr[k])
 
 
