@@ -1176,7 +1176,11 @@ void write_thermo_report(const Config& c, const std::filesystem::path& path) {
 // initialize :1761-1837
 using Observer = std::function<void(const Fields&, const char*, int)>;
 
-double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const Observer& obs) {
+SmoothReport smooth_device(Fields& F, const Config& c, int device);
+
+// smooth_device >= 0: the capillarity-only smoothing runs on that GPU
+// (pearl_smooth, blocked summation order) instead of on the host.
+double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const Observer& obs, int smooth_on = -1) {
     const int nx = c.nx, ny = c.ny;
     F.nx = nx;
     F.ny = ny;
@@ -1211,7 +1215,7 @@ double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const 
         }
     if (c.initialization == "sharp_smooth" || c.initialization == "microsim") {
         if (obs) obs(F, "init_sharp", 0);
-        SmoothReport r = smooth(F, c);
+        SmoothReport r = smooth_on >= 0 ? smooth_device(F, c, smooth_on) : smooth(F, c);
         if (rep) *rep = r;
         if (obs) obs(F, "init_smoothed", r.steps);
     } else if (rep) {
@@ -1248,6 +1252,42 @@ double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const 
 // ---------------------------------------------------------------------------
 void check(pearl_ctx* g, int st) {
     if (st != PEARL_OK) throw std::runtime_error(g ? pearl_last_error(g) : "pearl_gpu call failed");
+}
+
+// smooth_interface_only (:869-958) on the GPU: the sharp labels go up once,
+// pearl_smooth runs the pseudo-steps, the smoothed phases come back.  The
+// same controls and errors as the host version; the energies are summed in
+// the blocked order, so a marginal accept decision can differ from the
+// host's sequential sums.
+SmoothReport smooth_device(Fields& F, const Config& c, int device) {
+    if (c.smooth_steps < 1 || c.smooth_min_steps < 0 || c.smooth_min_steps > c.smooth_steps || !(c.smooth_energy_rtol > 0) ||
+        !(c.smooth_max_phase_change > 0) || c.smooth_max_phase_change > .2)
+        throw std::invalid_argument("invalid sharp-smoothing controls");
+    SmoothReport rep;
+    rep.before = phase_fractions(F);
+    pearl_ctx* g = nullptr;
+    check(nullptr, pearl_create(&c.ctl, c.initial_mu, PEARL_FP64, device, 0, &g));
+    pearl_smooth_report r{};
+    int st = PEARL_OK;
+    {
+        const std::vector<double> zero(F.n(), 0.0);  // carbon and mu play no part in the smoothing
+        pearl_scalars sc{};
+        st = pearl_set_state(g, F.phi.data(), zero.data(), zero.data(), &sc);
+    }
+    if (st == PEARL_OK)
+        st = pearl_smooth(g, c.smooth_steps, c.smooth_min_steps, c.smooth_energy_rtol, c.smooth_max_phase_change, &r);
+    if (st == PEARL_OK) st = pearl_get_state(g, F.phi.data(), nullptr, nullptr, nullptr);
+    const std::string msg = st == PEARL_OK ? "" : pearl_last_error(g);
+    pearl_destroy(g);
+    if (st != PEARL_OK) throw std::runtime_error(msg);
+    rep.steps = r.steps;
+    rep.converged = r.converged != 0;
+    rep.e0 = r.energy_initial;
+    rep.e1 = r.energy_final;
+    rep.rel = r.final_relative_energy_change;
+    rep.change = r.final_max_phase_change;
+    rep.after = phase_fractions(F);
+    return rep;
 }
 
 pearl_scalars to_scalars(const Meta& m) {
@@ -1289,11 +1329,16 @@ int main(int argc, char** argv) {
         std::string input = argv[1], output = argv[2], restart;
         bool thermo_only = false;
         int px = 1, py = 1, device = 0;
+        bool gpu_init = false;
         long long stop_steps = -1;
         for (int k = 3; k < argc; ++k) {
             const std::string key = argv[k];
             if (key == "--thermo-only") {
                 thermo_only = true;
+                continue;
+            }
+            if (key == "--gpu-init") {  // no reference counterpart: smoothing on the GPU
+                gpu_init = true;
                 continue;
             }
             if (k + 1 == argc) throw std::invalid_argument("missing CLI value");
@@ -1335,7 +1380,8 @@ int main(int argc, char** argv) {
             write_initial_phases(fl, config, std::filesystem::path(output) / (std::string(stage) + ".h5"), stage, pseudo);
         };
         const double mu_reference =
-            restart.empty() ? initialize(F, state, config, &rep, writer) : read_checkpoint(F, state, config, restart);
+            restart.empty() ? initialize(F, state, config, &rep, writer, gpu_init ? device : -1)
+                            : read_checkpoint(F, state, config, restart);
         if (restart.empty()) {
             std::ofstream log(std::filesystem::path(output) / "initialization_report.txt");
             log << std::setprecision(17);

@@ -135,3 +135,26 @@ def test_max_steps_stop(tmp_path):
     assert rc == 0, err
     st = read_status(os.path.join(tmp, "gpu", "STOPPED.txt"))
     assert st["status"] == "USER_STOPPED" and st["step"] == "3"
+
+
+def test_gpu_init_smoothing_matches_host(tmp_path):
+    """--gpu-init smooths the sharp labels on the GPU (pearl_smooth).  On
+    this input every accept decision agrees with the host's sequential
+    sums, so the initialized state (frame 0) is bitwise the host's; the
+    reported energies differ only by summation order."""
+    tmp = str(tmp_path)
+    inp = make_input(tmp, "colony")
+    rc, _, err = run(EXE, [inp, os.path.join(tmp, "cpu"), "--max-steps", "2"])
+    assert rc == 0, err
+    rc, _, err = run(EXE, [inp, os.path.join(tmp, "gpu"), "--max-steps", "2", "--gpu-init"])
+    assert rc == 0, err
+    a = load_checkpoint(inp, os.path.join(tmp, "cpu", "frame_000000.h5"))
+    b = load_checkpoint(inp, os.path.join(tmp, "gpu", "frame_000000.h5"))
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    ra = read_status(os.path.join(tmp, "cpu", "initialization_report.txt"))
+    rb = read_status(os.path.join(tmp, "gpu", "initialization_report.txt"))
+    for k in ["smooth_steps_completed", "smooth_energy_converged", "phase_fraction_after", "mu_reference"]:
+        assert ra[k] == rb[k], k
+    for k in ["smooth_energy_initial", "smooth_energy_final"]:
+        assert abs(float(ra[k]) - float(rb[k])) <= 1e-12 * abs(float(ra[k])), k
