@@ -32,3 +32,32 @@ def test_error_messages_match_reference_text():
     pkg = load_pkg()
     assert pkg.WHY == ["phase increment", "linear/closure", "composition/phase bounds", "mass conservation",
                        "free-energy increase"]
+
+
+def test_binding_structs_match_the_header(tmp_path):
+    """Every ctypes mirror of a C-ABI struct has the header's size and field
+    offsets (a C probe compiled with gcc prints them)."""
+    import shutil
+    import subprocess
+    pkg = load_pkg()
+    gcc = shutil.which("gcc")
+    assert gcc, "gcc is part of this image"
+    structs = {"pearl_params": pkg.Params, "pearl_scalars": pkg.Scalars, "pearl_step_info": pkg.StepInfo,
+               "pearl_diffusion_info": pkg.DiffusionInfo, "pearl_diag": pkg.Diag,
+               "pearl_smooth_report": pkg.SmoothReport}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pearl_gpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
