@@ -69,10 +69,11 @@ struct pearl_ctx {
     DevScal* hS = nullptr;  // pinned mirror
     std::string err;
     int cg_slots = 6;  // PCG slots per round
-    // graph of one round
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_slots = 0;
+    // graphs of one round: [0] without, [1] with the state-reduction kernels
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    int graph_slots[2] = {0, 0};
+    bool any_phase_only = false;  // a phase-only domain: rounds carry its kernels
     // profiling
     bool prof = false;
     long long launches[PEARL_KERNEL_COUNT]{};
@@ -148,7 +149,12 @@ hipEvent_t get_event(pearl_ctx* c) {
 // kernel and its finisher are bracketed by events (the cell kernel alone,
 // then the finisher) and the cell kernel writes its work record into
 // kinds[slot].
-void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide) {
+// One round of kernels.  The state reductions (e0, mass0 of a step) run only
+// in a round whose flag asks for them: their cached values only turn valid
+// during a run (host calls invalidate them), so a run needs them at most in
+// its first round.  The phase-only kernels are present only when a domain of
+// the context is phase-only.  A predicated-off launch costs several us.
+void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide, bool with_state) {
     const dim3 g(static_cast<unsigned>(c->total_blocks));
     const dim3 gd((c->ndom + 63) / 64);
     int slot = 0;
@@ -174,8 +180,9 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     hipStream_t st = c->stream;
     const Bufs& B = c->B;
     if (with_begin) hipLaunchKernelGGL(k_begin, gd, dim3(64), 0, st, B);
-    ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, B, s); },
-       [&] { hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B); });
+    if (with_state)
+        ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, B, s); },
+           [&] { hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B); });
     ev(PEARL_KERNEL_PROPOSE, [&](int s) { PEARL_LAUNCH(c, k_propose, g, B, s); },
        [&] { hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B); });
     ev(PEARL_KERNEL_PREPARE, [&](int s) { PEARL_LAUNCH_P(c, k_prepare, g, B, s); },
@@ -188,8 +195,9 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     }
     ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, B, s); },
        [&] { hipLaunchKernelGGL(k_finalize_fin, gf, tb, 0, st, B); });
-    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
-       [&] { hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B); });
+    if (c->any_phase_only)
+        ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
+           [&] { hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B); });
     if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, st, B);
 }
 
@@ -251,11 +259,12 @@ int write_scalars(pearl_ctx* c) {
 
 // Replay `rounds` rounds: from the captured graph, or eagerly with events
 // (profiling; the work records are read after every round).
-int run_rounds(pearl_ctx* c, int rounds) {
+// `rounds` rounds; the first carries the state reductions when `state_first`.
+int run_rounds(pearl_ctx* c, int rounds, bool state_first) {
     if (c->prof) {
         for (int r = 0; r < rounds; ++r) {
             hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
-            enqueue_round(c, c->cg_slots, true, true, true);
+            enqueue_round(c, c->cg_slots, true, true, true, state_first && r == 0);
             if (r + 1 < rounds) {
                 const int st = read_scalars(c);
                 if (st) return st;
@@ -263,18 +272,21 @@ int run_rounds(pearl_ctx* c, int rounds) {
         }
         return PEARL_OK;
     }
-    if (!c->graph_exec || c->graph_slots != c->cg_slots) {
-        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-        if (c->graph) (void)hipGraphDestroy(c->graph);
-        c->graph_exec = nullptr;
-        c->graph = nullptr;
+    for (int v = 0; v < 2; ++v) {
+        if (v == 1 && !state_first) continue;
+        if (c->graph_exec[v] && c->graph_slots[v] == c->cg_slots) continue;
+        if (c->graph_exec[v]) (void)hipGraphExecDestroy(c->graph_exec[v]);
+        if (c->graph[v]) (void)hipGraphDestroy(c->graph[v]);
+        c->graph_exec[v] = nullptr;
+        c->graph[v] = nullptr;
         HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_round(c, c->cg_slots, false, true, true);
-        HIP_CHECK(c, hipStreamEndCapture(c->stream, &c->graph));
-        HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        c->graph_slots = c->cg_slots;
+        enqueue_round(c, c->cg_slots, false, true, true, v == 1);
+        HIP_CHECK(c, hipStreamEndCapture(c->stream, &c->graph[v]));
+        HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec[v], c->graph[v], nullptr, nullptr, 0));
+        c->graph_slots[v] = c->cg_slots;
     }
-    for (int r = 0; r < rounds; ++r) HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    for (int r = 0; r < rounds; ++r)
+        HIP_CHECK(c, hipGraphLaunch(c->graph_exec[state_first && r == 0 ? 1 : 0], c->stream));
     return PEARL_OK;
 }
 
@@ -356,9 +368,19 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
         }
         return std::min<long long>(std::max<long long>(best, 1), 64);
     };
+    // the state reductions are needed only if a starting domain has no
+    // cached e0 / mass0 (k_begin's need_state); they then run in round one
+    bool state_first = false;
+    for (int d = 0; d < c->ndom; ++d) {
+        const DevScal& h = c->hS[d];
+        if (h.phase == PH_IDLE) continue;
+        const KP& P = c->kp[d];
+        if ((P.check_energy && !h.e0_valid) || (!P.phase_only && !h.m0_valid)) state_first = true;
+    }
     long long batch = rounds_left();
     while (true) {
-        st = run_rounds(c, static_cast<int>(batch));
+        st = run_rounds(c, static_cast<int>(batch), state_first);
+        state_first = false;
         if (st) return st;
         st = read_scalars(c);
         if (st) return st;
@@ -516,6 +538,7 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
     c->k = k;
     c->mx = precision == PEARL_MIXED ? 1 : 0;
     c->prm.assign(params, params + count);
+    for (int d = 0; d < count; ++d) c->any_phase_only = c->any_phase_only || params[d].phase_only != 0;
     c->sc.assign(count, pearl_scalars{});
     c->runs_seen.assign(static_cast<size_t>(count) * NKIND, 0u);
     // Pack the domains; each slice starts on a 64-cell (512-byte) boundary.
@@ -604,8 +627,10 @@ int pearl_destroy(pearl_ctx* c) {
         (void)hipEventDestroy(e.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (int v = 0; v < 2; ++v) {
+        if (c->graph_exec[v]) (void)hipGraphExecDestroy(c->graph_exec[v]);
+        if (c->graph[v]) (void)hipGraphDestroy(c->graph[v]);
+    }
     if (c->mem) (void)hipFree(c->mem);
     if (c->d_kp) (void)hipFree(c->d_kp);
     if (c->d_S) (void)hipFree(c->d_S);
@@ -882,9 +907,9 @@ int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, d
     for (int guard = 0; guard < 100000; ++guard) {
         if (c->prof) {
             hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
-            enqueue_round(c, c->cg_slots, true, false, false);
+            enqueue_round(c, c->cg_slots, true, false, false, false);
         } else {
-            enqueue_round(c, c->cg_slots, false, false, false);
+            enqueue_round(c, c->cg_slots, false, false, false, false);
         }
         st = read_scalars(c);
         if (st) return st;
@@ -948,7 +973,7 @@ int step_fixed(pearl_ctx* c, double dt) {
     st = write_scalars(c);
     if (st) return st;
     for (int guard = 0; guard < 100000; ++guard) {
-        enqueue_round(c, c->cg_slots, false, false, false);
+        enqueue_round(c, c->cg_slots, false, false, false, false);
         st = read_scalars(c);
         if (st) return st;
         if (h.phase != PH_SOLVE) break;
