@@ -111,7 +111,8 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
-           "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance"]
+           "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance",
+           "pearl_run_fixed"]
 
 
 def lib():
@@ -137,6 +138,7 @@ def lib():
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
         L.pearl_step_fixed.argtypes = [vp, C.c_double]
         L.pearl_set_stop_tolerance.argtypes = [vp, C.c_double]
+        L.pearl_run_fixed.argtypes = [vp, C.c_int, _dp]
         L.pearl_adjoint.argtypes = [vp, C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
         L.pearl_propose_phi.argtypes = [vp, C.c_double, _dp, _dp]
         L.pearl_implicit_diffusion.argtypes = [vp, _dp, C.c_double, _dp, _dp, C.POINTER(DiffusionInfo)]
@@ -297,6 +299,12 @@ class Solver:
     def step_fixed(self, dt: float):
         """One always-taken step of size dt (propose_phi, implicit_diffusion)."""
         self._check(lib().pearl_step_fixed(self._h, float(dt)))
+
+    def run_fixed(self, dts):
+        """len(dts) always-taken steps of the given sizes in one call
+        (pearl_run_fixed); the same result as step_fixed per size."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(lib().pearl_run_fixed(self._h, int(dts.size), _ptr(dts)))
 
     def adjoint(self, dts, phi_target, every: int = 100, w_front: float = 0.0, front_target: float = 0.0):
         """Loss and gradient (ADJOINT_GRAD order: sigma, tau_at, tau_ag,

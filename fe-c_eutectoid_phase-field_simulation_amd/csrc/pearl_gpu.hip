@@ -58,6 +58,9 @@ struct pearl_ctx {
     std::vector<KP> kp;             // host copy of the parameter blocks
     std::vector<pearl_scalars> sc;  // host-side copy of the State scalars
     double stop_eps = 0;            // pearl_set_stop_tolerance
+    double* d_dt1 = nullptr;        // device copy of pearl_step_fixed's dt
+    double* d_dtn = nullptr;        // device copy of pearl_run_fixed's dts
+    long long d_dtn_cap = 0;
     long long total_cells = 0, total_blocks = 0, total_parts = 0;
     // device memory
     double* mem = nullptr;
@@ -69,10 +72,11 @@ struct pearl_ctx {
     DevScal* hS = nullptr;  // pinned mirror
     std::string err;
     int cg_slots = 6;  // PCG slots per round
-    // graphs of one round: [0] without, [1] with the state-reduction kernels
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-    int graph_slots[2] = {0, 0};
+    // graphs of one round: [0] plain, [1] with the state-reduction kernels,
+    // [2] with the fixed-run state copy (adjoint segment recompute)
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
+    int graph_slots[3] = {0, 0, 0};
     bool any_phase_only = false;  // a phase-only domain: rounds carry its kernels
     // profiling
     bool prof = false;
@@ -154,7 +158,8 @@ hipEvent_t get_event(pearl_ctx* c) {
 // during a run (host calls invalidate them), so a run needs them at most in
 // its first round.  The phase-only kernels are present only when a domain of
 // the context is phase-only.  A predicated-off launch costs several us.
-void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide, bool with_state) {
+void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide, bool with_state,
+                   bool with_save = false) {
     const dim3 g(static_cast<unsigned>(c->total_blocks));
     const dim3 gd((c->ndom + 63) / 64);
     int slot = 0;
@@ -199,6 +204,11 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
         ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
            [&] { hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B); });
     if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, st, B);
+    if (with_save) {
+        hipLaunchKernelGGL(k_fixed_save, dim3(static_cast<unsigned>(std::min<long long>(c->total_blocks, 2048))), tb, 0,
+                           st, B);
+        hipLaunchKernelGGL(k_fixed_saved, dim3(1), dim3(64), 0, st, B);
+    }
 }
 
 // Attribute the timed launches since the last harvest (stream idle) and
@@ -259,12 +269,13 @@ int write_scalars(pearl_ctx* c) {
 
 // Replay `rounds` rounds: from the captured graph, or eagerly with events
 // (profiling; the work records are read after every round).
-// `rounds` rounds; the first carries the state reductions when `state_first`.
-int run_rounds(pearl_ctx* c, int rounds, bool state_first) {
+// `rounds` rounds; the first carries the state reductions when `state_first`;
+// with `save` every round ends with the fixed-run state copy.
+int run_rounds(pearl_ctx* c, int rounds, bool state_first, bool save = false) {
     if (c->prof) {
         for (int r = 0; r < rounds; ++r) {
             hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
-            enqueue_round(c, c->cg_slots, true, true, true, state_first && r == 0);
+            enqueue_round(c, c->cg_slots, true, true, true, state_first && r == 0, save);
             if (r + 1 < rounds) {
                 const int st = read_scalars(c);
                 if (st) return st;
@@ -272,21 +283,20 @@ int run_rounds(pearl_ctx* c, int rounds, bool state_first) {
         }
         return PEARL_OK;
     }
-    for (int v = 0; v < 2; ++v) {
-        if (v == 1 && !state_first) continue;
+    const int first = save ? 2 : (state_first ? 1 : 0), rest = save ? 2 : 0;
+    for (int v : {first, rest}) {
         if (c->graph_exec[v] && c->graph_slots[v] == c->cg_slots) continue;
         if (c->graph_exec[v]) (void)hipGraphExecDestroy(c->graph_exec[v]);
         if (c->graph[v]) (void)hipGraphDestroy(c->graph[v]);
         c->graph_exec[v] = nullptr;
         c->graph[v] = nullptr;
         HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_round(c, c->cg_slots, false, true, true, v == 1);
+        enqueue_round(c, c->cg_slots, false, true, true, v == 1, v == 2);
         HIP_CHECK(c, hipStreamEndCapture(c->stream, &c->graph[v]));
         HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec[v], c->graph[v], nullptr, nullptr, 0));
         c->graph_slots[v] = c->cg_slots;
     }
-    for (int r = 0; r < rounds; ++r)
-        HIP_CHECK(c, hipGraphLaunch(c->graph_exec[state_first && r == 0 ? 1 : 0], c->stream));
+    for (int r = 0; r < rounds; ++r) HIP_CHECK(c, hipGraphLaunch(c->graph_exec[r == 0 ? first : rest], c->stream));
     return PEARL_OK;
 }
 
@@ -300,6 +310,7 @@ int status_of(pearl_ctx* c, int d) {
         case ST_RETRY_LIMIT: return fail(c, PEARL_ERR_RETRY_LIMIT, std::string("step retry limit: ") + kWhy[why]);
         case ST_MIXTURE: return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
         case ST_PROJECTION: return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+        case ST_FIXED_SOLVE: return fail(c, PEARL_ERR_ARGUMENT, "fixed step: the linear solve did not converge");
         default: return fail(c, PEARL_ERR_DEVICE, "unknown device status");
     }
 }
@@ -627,11 +638,13 @@ int pearl_destroy(pearl_ctx* c) {
         (void)hipEventDestroy(e.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    for (int v = 0; v < 2; ++v) {
+    for (int v = 0; v < 3; ++v) {
         if (c->graph_exec[v]) (void)hipGraphExecDestroy(c->graph_exec[v]);
         if (c->graph[v]) (void)hipGraphDestroy(c->graph[v]);
     }
     if (c->mem) (void)hipFree(c->mem);
+    if (c->d_dt1) (void)hipFree(c->d_dt1);
+    if (c->d_dtn) (void)hipFree(c->d_dtn);
     if (c->d_kp) (void)hipFree(c->d_kp);
     if (c->d_S) (void)hipFree(c->d_S);
     if (c->d_blk2dom) (void)hipFree(c->d_blk2dom);
@@ -938,69 +951,58 @@ int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, d
 }
 
 namespace {
-// One step of size dt that is always taken: propose_phi (:961-1002), then
-// implicit_diffusion (:1010-1094) on the trial phases, then the trial state
-// becomes current.  No acceptance test, no dt control: the differentiable
-// map that the adjoint inverts.  Domain 0, coupled mode, FP64.
-int step_fixed(pearl_ctx* c, double dt) {
+// `n` steps of the sizes dts[0..n-1] (a device array) that are always
+// taken: propose_phi (:961-1002), then implicit_diffusion (:1010-1094) on the
+// trial phases, then the trial state becomes current.  No acceptance test,
+// no dt control: the differentiable map that the adjoint inverts.  Domain
+// 0, coupled mode.  The steps run under the device controller in fixed-dt
+// mode, from replayed rounds, with a synchronisation per batch of rounds.
+// With `save` (device memory for n states of 5 planes), the state after
+// step k is copied to save + (k-1)*5n as the run goes.
+int run_fixed(pearl_ctx* c, const double* dts, int n, double* save = nullptr) {
     int st = read_scalars(c);
     if (st) return st;
     DevScal& h = c->hS[0];
-    h.phase = PH_PROPOSE;
-    h.need_state = 0;
-    h.dt = dt;
     h.status = ST_OK;
-    h.err = 0;
+    h.why = -1;
+    h.taken = 0;
+    h.target_steps = n;
+    h.fixed = 1;
+    h.fixed_dts = dts;
+    h.fixed_save = save;
+    h.fixed_saved = 0;
+    h.cg_peak = 0;
+    h.phase = PH_START;
     st = write_scalars(c);
     if (st) return st;
-    PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
-    hipLaunchKernelGGL(k_propose_fin, dim3(static_cast<unsigned>(c->ndom)), dim3(TPB), 0, c->stream, c->B);
-    st = read_scalars(c);
-    if (st) return st;
-    if (h.status == ST_PROJECTION) {
-        h.phase = PH_IDLE;
-        h.status = ST_OK;
-        (void)write_scalars(c);
-        return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
-    }
-    h.phase = PH_SOLVE;
-    h.live = 1;
-    h.cg_state = CG_IDLE;
-    h.cg_it = 0;
-    h.cg_res = 0;
-    h.closure = 0;
-    h.pad = 0;
-    st = write_scalars(c);
-    if (st) return st;
-    for (int guard = 0; guard < 100000; ++guard) {
-        enqueue_round(c, c->cg_slots, false, false, false, false);
+    long long batch = std::min(n, 64);
+    while (true) {
+        st = run_rounds(c, static_cast<int>(batch), false, save != nullptr);
+        if (st) return st;
         st = read_scalars(c);
         if (st) return st;
-        if (h.phase != PH_SOLVE) break;
+        if (h.phase == PH_IDLE) break;
+        batch = std::max<long long>(1, std::min<long long>(64, h.target_steps - h.taken));
     }
-    if (h.status != ST_OK) {
-        const int code = status_of(c, 0);
-        h.phase = PH_IDLE;
-        h.status = ST_OK;
-        (void)write_scalars(c);
-        return code;
-    }
-    if (h.cg_state != CG_OK) {
-        h.phase = PH_IDLE;
-        (void)write_scalars(c);
-        return fail(c, PEARL_ERR_ARGUMENT, "fixed step: the linear solve did not converge");
-    }
-    h.cur ^= 1;
-    h.time = h.time + dt;
-    h.step = h.step + 1;
-    h.e0_valid = 0;
-    h.m0_valid = 0;
+    const int code = h.status == ST_OK ? PEARL_OK : status_of(c, 0);
+    if (h.cg_peak > 0) c->cg_slots = std::min(24, std::max(3, h.cg_peak + 2));
+    h.fixed = 0;
+    h.fixed_dts = nullptr;
+    h.fixed_save = nullptr;
+    h.status = ST_OK;
     h.phase = PH_IDLE;
     st = write_scalars(c);
     if (st) return st;
     c->sc[0].time = h.time;
     c->sc[0].step = h.step;
-    return PEARL_OK;
+    return code;
+}
+
+int step_fixed(pearl_ctx* c, double dt) {
+    if (!c->d_dt1 && hipMalloc(&c->d_dt1, sizeof(double)) != hipSuccess)
+        return fail(c, PEARL_ERR_ALLOC, "out of device memory");
+    HIP_CHECK(c, hipMemcpyAsync(c->d_dt1, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return run_fixed(c, c->d_dt1, 1);
 }
 }  // namespace
 
@@ -1009,6 +1011,24 @@ int pearl_step_fixed(pearl_ctx* c, double dt) {
     if (st) return st;
     if (c->prm[0].phase_only || c->mx || !(dt > 0) || !std::isfinite(dt)) return PEARL_ERR_ARGUMENT;
     return step_fixed(c, dt);
+}
+
+int pearl_run_fixed(pearl_ctx* c, int steps, const double* dts) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (c->ndom != 1 || c->prm[0].phase_only || c->mx || steps < 1 || !dts) return PEARL_ERR_ARGUMENT;
+    for (int k = 0; k < steps; ++k)
+        if (!(dts[k] > 0) || !std::isfinite(dts[k])) return PEARL_ERR_ARGUMENT;
+    if (steps > c->d_dtn_cap) {
+        if (c->d_dtn) (void)hipFree(c->d_dtn);
+        c->d_dtn = nullptr;
+        c->d_dtn_cap = 0;
+        if (hipMalloc(&c->d_dtn, steps * sizeof(double)) != hipSuccess)
+            return fail(c, PEARL_ERR_ALLOC, "out of device memory");
+        c->d_dtn_cap = steps;
+    }
+    HIP_CHECK(c, hipMemcpyAsync(c->d_dtn, dts, steps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return run_fixed(c, c->d_dtn, steps);
 }
 
 namespace {
@@ -1040,17 +1060,19 @@ int adjoint_solve(pearl_ctx* c, const Adj& A) {
     const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
     PEARL_LAUNCH(c, k_adj_prepare, grid, c->B, A);
     hipLaunchKernelGGL(k_adj_prepare_fin, one, tb, 0, c->stream, c->B);
+    // PCG slots, then one synchronisation to see whether the solve is done
+    // (slots after convergence are predicated off)
     for (int guard = 0; guard < 10000; ++guard) {
-        st = read_scalars(c);
-        if (st) return st;
-        if (h.cg_state == CG_OK) break;
-        if (h.cg_state == CG_FAIL) return fail(c, PEARL_ERR_ARGUMENT, "adjoint: the linear solve did not converge");
         for (int k = 0; k < c->cg_slots; ++k) {
             PEARL_LAUNCH_P(c, k_cg_a, grid, c->B, -1);
             hipLaunchKernelGGL(k_cg_a_fin, one, tb, 0, c->stream, c->B);
             PEARL_LAUNCH_P(c, k_cg_b, grid, c->B, -1);
             hipLaunchKernelGGL(k_cg_b_fin, one, tb, 0, c->stream, c->B);
         }
+        st = read_scalars(c);
+        if (st) return st;
+        if (h.cg_state == CG_OK) break;
+        if (h.cg_state == CG_FAIL) return fail(c, PEARL_ERR_ARGUMENT, "adjoint: the linear solve did not converge");
     }
     h.phase = PH_IDLE;
     return write_scalars(c);
@@ -1073,7 +1095,8 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
     // device memory: checkpoints, the states of one segment, the final state,
     // two gradient sets, the trial-phase gradient, v and the accumulators
     const size_t state = 5 * n;
-    const size_t total = state * (static_cast<size_t>(segs) + seg_len + 1 + 1) + 2 * state + 6 * n + ADJ_THETA;
+    const size_t total =
+        state * (static_cast<size_t>(segs) + seg_len + 1 + 1) + 2 * state + 6 * n + ADJ_THETA + static_cast<size_t>(steps);
     double* mem = nullptr;
     if (hipMalloc(&mem, total * sizeof(double)) != hipSuccess) return fail(c, PEARL_ERR_ALLOC, "adjoint: out of memory");
     struct Free {
@@ -1087,6 +1110,7 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
     double* gphit = G[1] + state;
     double* vbuf = gphit + 3 * n;
     double* theta = vbuf + 3 * n;
+    double* d_dts = theta + ADJ_THETA;
     auto ph = [&](double* s5) { return s5; };
     auto cc = [&](double* s5) { return s5 + 3 * n; };
     auto mm = [&](double* s5) { return s5 + 4 * n; };
@@ -1099,11 +1123,13 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
     auto load = [&](double* s5) {
         return copy_state(c, dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur), ph(s5), cc(s5), mm(s5));
     };
-    // forward with checkpoints
+    HIP_CHECK(c, hipMemcpyAsync(d_dts, dts, steps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // forward with checkpoints: one device-controlled run per segment
     if ((st = save(ck))) return st;
-    for (int k = 0; k < steps; ++k) {
-        if ((st = step_fixed(c, dts[k]))) return st;
-        if ((k + 1) % every == 0 && (k + 1) < steps && (st = save(ck + state * ((k + 1) / every)))) return st;
+    for (int sgi = 0; sgi < segs; ++sgi) {
+        const int s0 = sgi * every, len = std::min(every, steps - s0);
+        if ((st = run_fixed(c, d_dts + s0, len))) return st;
+        if (s0 + len < steps && (st = save(ck + state * (sgi + 1)))) return st;
     }
     if ((st = save(fin))) return st;
     const double t_end = h.time;
@@ -1143,10 +1169,7 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
         const int s0 = sgi * every, len = std::min(every, steps - s0);
         if ((st = load(ck + state * sgi))) return st;
         if ((st = save(seg))) return st;
-        for (int k = 0; k < len; ++k) {
-            if ((st = step_fixed(c, dts[s0 + k]))) return st;
-            if ((st = save(seg + state * (k + 1)))) return st;
-        }
+        if ((st = run_fixed(c, d_dts + s0, len, seg + state))) return st;
         for (int k = len - 1; k >= 0; --k) {
             double* s_n = seg + state * k;
             double* s_n1 = seg + state * (k + 1);

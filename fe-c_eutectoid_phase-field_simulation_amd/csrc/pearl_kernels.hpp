@@ -71,7 +71,8 @@ enum { PH_IDLE = 0, PH_START = 1, PH_PROPOSE = 2, PH_SOLVE = 3, PH_DECIDE = 4 };
 // CG_IDLE: not started.
 enum { CG_IDLE = 0, CG_ITER = 1, CG_TRUE = 2, CG_OK = 3, CG_FAIL = 4 };
 // Device error codes (the C ABI's PEARL_ERR_* values).
-enum { ST_OK = 0, ST_INVALID_TIME = 1, ST_DT_MIN = 2, ST_RETRY_LIMIT = 3, ST_MIXTURE = 4, ST_PROJECTION = 5 };
+enum { ST_OK = 0, ST_INVALID_TIME = 1, ST_DT_MIN = 2, ST_RETRY_LIMIT = 3, ST_MIXTURE = 4, ST_PROJECTION = 5,
+       ST_FIXED_SOLVE = 6 };
 enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4, ERRF_MIXTURE_STATE = 8 };
 // Kernels counted per domain (work actually done).
 enum { RUN_PROPOSE = 0, RUN_PREPARE, RUN_CG_APPLY, RUN_CG_UPDATE, RUN_TRUE_RESIDUAL, RUN_FINALIZE, RUN_STATE,
@@ -109,6 +110,11 @@ struct DevScal {
     // counters
     long long attempts, rejected_attempts, cg_all, cg_total;
     int cg_peak, pad;
+    // fixed-dt mode (the differentiated map of the adjoint): step k takes
+    // dt = fixed_dts[k] and is accepted whenever its solve converged
+    int fixed, fixed_saved;  // fixed_saved: steps already copied to fixed_save
+    const double* fixed_dts;
+    double* fixed_save;      // if set: state after fixed step k -> fixed_save + (k-1)*5n
     unsigned runs[NKIND];
 };
 
@@ -566,9 +572,14 @@ __global__ void k_begin(Bufs B) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
     if (S->phase == PH_START) {
-        double dt = S->dt_state;
-        if (P.dt_max < dt) dt = P.dt_max;
-        if (S->stop_time - S->time < dt) dt = S->stop_time - S->time;
+        double dt;
+        if (S->fixed) {
+            dt = S->fixed_dts[S->taken];
+        } else {
+            dt = S->dt_state;
+            if (P.dt_max < dt) dt = P.dt_max;
+            if (S->stop_time - S->time < dt) dt = S->stop_time - S->time;
+        }
         if (!(dt > 0) || !__builtin_isfinite(dt)) {
             S->status = ST_INVALID_TIME;
             S->phase = PH_IDLE;
@@ -577,7 +588,8 @@ __global__ void k_begin(Bufs B) {
         S->dt = dt;
         S->attempt = 0;
         S->phase = PH_PROPOSE;
-        S->need_state = ((P.check_energy && !S->e0_valid) || (!P.phase_only && !S->m0_valid)) ? 1 : 0;
+        S->need_state =
+            (!S->fixed && ((P.check_energy && !S->e0_valid) || (!P.phase_only && !S->m0_valid))) ? 1 : 0;
     }
     if (S->phase == PH_PROPOSE) {
         S->live = 0;
@@ -766,7 +778,8 @@ __global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
         return;
     }
     // :1131 -- the rest of the attempt runs only if the increment passes
-    S->live = (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change) ? 1 : 0;
+    // (a fixed-dt step always continues)
+    S->live = (S->fixed || (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change)) ? 1 : 0;
     S->phase = S->live ? PH_SOLVE : PH_DECIDE;
 }
 
@@ -1207,6 +1220,24 @@ __global__ void k_decide(Bufs B) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
     if (S->phase != PH_DECIDE) return;
+    if (S->fixed) {
+        // a fixed-dt step: taken whenever the solve converged, no acceptance
+        // tests, no dt control, the cached e0/mass0 dropped
+        if (S->cg_state != CG_OK) {
+            S->status = ST_FIXED_SOLVE;
+            S->phase = PH_IDLE;
+            return;
+        }
+        S->cur ^= 1;
+        S->time = S->time + S->dt;
+        S->step = S->step + 1;
+        S->e0_valid = 0;
+        S->m0_valid = 0;
+        if (S->cg_it > S->cg_peak) S->cg_peak = S->cg_it;
+        S->taken++;
+        S->phase = S->taken >= S->target_steps ? PH_IDLE : PH_START;
+        return;
+    }
     bool good = S->live != 0;
     int why = 0;  // phase increment
     double residual = 0, closure = 0;
@@ -1279,6 +1310,33 @@ __global__ void k_decide(Bufs B) {
         S->attempt++;
         S->phase = PH_PROPOSE;
     }
+}
+
+// Fixed-dt runs that keep every state (the adjoint's segment recompute):
+// after each taken step, the current phi (3 planes), c and mu of domain 0
+// are copied to the next slot of fixed_save.  Grid-stride over 5n values.
+__global__ __launch_bounds__(TPB) void k_fixed_save(Bufs B) {
+    const KP P = B.kp[0];
+    DevScal* S = B.S;
+    if (!S->fixed || !S->fixed_save || S->fixed_saved >= S->taken) return;
+    const long long n = P.n, k = S->taken;
+    double* dst = S->fixed_save + (k - 1) * 5 * n;
+    const int cur = S->cur;
+    const double* phi = B.phi[cur] + 3 * P.off;
+    const double* c = B.c[cur] + P.off;
+    const double* mu = B.mu[cur] + P.off;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < 5 * n; i += stride) {
+        const double v = i < 3 * n ? phi[i] : (i < 4 * n ? c[i - 3 * n] : mu[i - 4 * n]);
+        st_once(dst + i, v);
+    }
+}
+
+// Marks the step copied by k_fixed_save (a separate launch, so no block of
+// the copy can see the mark before the copy is done).
+__global__ void k_fixed_saved(Bufs B) {
+    DevScal* S = B.S;
+    if (threadIdx.x == 0 && S->fixed && S->fixed_save) S->fixed_saved = static_cast<int>(S->taken);
 }
 
 __global__ void k_clear_kinds(int* kinds, int n) {

@@ -100,8 +100,7 @@ class FixedStepProblem:
     def final_phases(self, theta) -> np.ndarray:
         """Forward only: the phases after the fixed steps."""
         with self._solver(theta) as s:
-            for dt in self.dts:
-                s.step_fixed(float(dt))
+            s.run_fixed(self.dts)
             return s.get_state()[0]
 
     def loss_of_phases(self, phi: np.ndarray) -> float:

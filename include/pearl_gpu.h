@@ -191,6 +191,13 @@ int pearl_smooth(pearl_ctx* ctx, int max_steps, int min_steps, double energy_rto
  * FP64, domain 0.  No reference counterpart. */
 int pearl_step_fixed(pearl_ctx* ctx, double dt);
 
+/* `steps` such steps of sizes dts[0..steps-1] in one call: the device
+ * controller runs them from replayed rounds (one synchronisation per batch
+ * of rounds), with the same kernels and results as that many
+ * pearl_step_fixed calls.  Single-domain FP64 coupled contexts.  No
+ * reference counterpart. */
+int pearl_run_fixed(pearl_ctx* ctx, int steps, const double* dts);
+
 /* Gradient of the loss
  *   L = 1/2 sum_cells,phases (phi_N - phi_target)^2 + 1/2 w_front (H - H_target)^2,
  *   H = dy/nx * sum_cells (1 - phi_gamma)   (the transformed height),

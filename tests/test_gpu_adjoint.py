@@ -195,3 +195,19 @@ def test_autograd_function_harness(pkg, colony):
     pred = -a * float(np.sum((t * g0) ** 2))
     say(f"  descent: dL {dL:.6e} predicted {pred:.6e}")
     assert dL < 0 and abs(dL - pred) <= 1e-2 * abs(pred)
+
+
+def test_run_fixed_equals_step_fixed(pkg):
+    """pearl_run_fixed (all steps under the device controller, one call) is
+    bitwise the same as one pearl_step_fixed call per size."""
+    g, pd, mu_ref, dts, *_ = setup(PAIRS)
+    dts = dts * np.linspace(0.5, 1.0, dts.size)
+    with pkg.Solver(pd, mu_ref) as a, pkg.Solver(pd, mu_ref) as b:
+        for s in (a, b):
+            s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        for dt in dts:
+            a.step_fixed(float(dt))
+        b.run_fixed(dts)
+        for x, y in zip(a.get_state()[:3], b.get_state()[:3]):
+            assert np.array_equal(x, y)
+        assert a.scalars().time == b.scalars().time and a.scalars().step == b.scalars().step
