@@ -317,6 +317,79 @@ def run_ensemble(args, pkg):
     print(json.dumps(out), flush=True)
 
 
+def run_smoke(args, pkg):
+    """Config 0 end to end: the reference's own CPU case (256x128, 4 equal
+    lamella pairs of 8 cementite + 56 ferrite columns in rows 0-15, 10 K, to
+    0.015 us, adaptive dt), run by the drop-in program pearl_gpu and by the
+    unmodified reference programs (compiled from source under oracle/_ref)
+    on the same input file: pearl_serial on one core and pearl_parallel under
+    mpiexec (16 ranks, 4x4).  Each program's step loop is timed from its own
+    diagnostics.csv (wall_s of the last output row); both include the
+    output-time frames.  Reports the step, rejection and PCG counts of both
+    and the final-state differences."""
+    import importlib
+    import subprocess
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_tool
+    inputs = importlib.import_module("pearl_amd.inputs")
+    tmp = tempfile.mkdtemp()
+    inp = os.path.join(tmp, "smoke.in")
+    inputs.colony_input(inp, 256, 128, 4, 16, 1.5e-8, output_dt=1.5e-9)
+    exe = os.path.join(PKG_DIR, "pearl_gpu")
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+
+    def run(cmd, out, timeout):
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError(f"{cmd[0]} failed: {r.stderr.strip()[-500:]}")
+        rows = np.genfromtxt(os.path.join(out, "diagnostics.csv"), delimiter=",", names=True)
+        status = dict(l.split("=", 1) for l in open(os.path.join(out, "COMPLETED.txt")).read().splitlines() if "=" in l)
+        steps = int(rows["step"][-1])
+        loop = float(rows["wall_s"][-1])
+        log(f"  {os.path.basename(cmd[0])}: {steps} steps, loop {loop:.2f} s, whole run {time.perf_counter() - t0:.2f} s")
+        return dict(steps=steps, loop_s=loop, rejected=int(status["rejected"]), cg_total=int(status["CG_total"]),
+                    value=256 * 128 * steps / loop / 1e6)
+
+    log("smoke (config 0): drop-in program on the GPU")
+    gpu = run([exe, inp, os.path.join(tmp, "gpu")], os.path.join(tmp, "gpu"), 600)
+    out = {"metric": "Mcell-steps/s", "value": round(gpu["value"], 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+           "steps": gpu["steps"], "warmup": 0, "ms_per_step": round(gpu["loop_s"] * 1e3 / gpu["steps"], 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "config 0: 256x128, 4 equal lamella pairs (8 cementite + 56 ferrite columns, "
+                                  "rows 0-15), 10 K, to 0.015 us, adaptive dt; the drop-in program pearl_gpu end to "
+                                  "end, step loop from its diagnostics.csv (frames included)",
+                      "nx": 256, "ny": 128},
+           "gpu": gpu}
+    serial = os.path.join(ref_dir, "pearl_serial")
+    if os.path.exists(serial) and not args.no_cpu_baseline:
+        log("smoke: reference pearl_serial")
+        ref = run([serial, inp, os.path.join(tmp, "ref")], os.path.join(tmp, "ref"), 1200)
+        out["cpu_baseline_serial"] = dict(value=ref["value"], unit="Mcell-steps/s", cores=1, kind="reference",
+                                          sample=f"the whole config-0 run, {ref['steps']} steps, loop "
+                                                 f"{ref['loop_s']:.1f} s", **{k: ref[k] for k in ("steps", "rejected",
+                                                                                                   "cg_total")})
+        a = ref_tool.read_checkpoint(inp, os.path.join(tmp, "ref", "checkpoint.h5"), tmp)
+        b = ref_tool.read_checkpoint(inp, os.path.join(tmp, "gpu", "checkpoint.h5"), tmp)
+        out["parity"] = {"same_step_rejection_cg_counts": (ref["steps"], ref["rejected"], ref["cg_total"]) ==
+                         (gpu["steps"], gpu["rejected"], gpu["cg_total"]),
+                         "max_abs_dmu_J_per_mol": float(np.max(np.abs(a[2] - b[2]))),
+                         "max_abs_dphi": float(np.max(np.abs(a[0] - b[0])))}
+        par, mpi = ref_tool.parallel_program(), ref_tool.mpiexec()
+        if par and mpi:
+            px, py = ref_tool.grid_split(16, 256, 128)
+            log("smoke: reference pearl_parallel, 16 ranks")
+            m = run([mpi, "-n", "16", par, inp, os.path.join(tmp, "mpi"), "--px", str(px), "--py", str(py)],
+                    os.path.join(tmp, "mpi"), 1200)
+            out["cpu_baseline"] = dict(value=m["value"], unit="Mcell-steps/s", cores=16, kind="reference",
+                                       sample=f"the whole config-0 run under mpiexec -n 16 ({px}x{py}), "
+                                              f"{m['steps']} steps, loop {m['loop_s']:.1f} s")
+        else:
+            out["cpu_baseline"] = out["cpu_baseline_serial"]
+    print(json.dumps(out), flush=True)
+
+
 def run_adjoint(args, pkg):
     """The checkpointed adjoint on the parameter-recovery case: 512x256, 4
     lamella pairs, 15 K, a frozen dt sequence of --adj-steps steps,
@@ -414,7 +487,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the second region with per-kernel events (no roofline)")
-    ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy", "adjoint"], default="production")
+    ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy", "adjoint", "smoke"],
+                    default="production")
     ap.add_argument("--shift-trigger", type=float, default=None,
                     help="production: moving-window trigger as a fraction of the domain height (default: the "
                          "input default 0.65, which this colony does not reach)")
@@ -456,6 +530,9 @@ def main():
         return
     if args.workload == "adjoint":
         run_adjoint(args, pkg)
+        return
+    if args.workload == "smoke":
+        run_smoke(args, pkg)
         return
     try:
         import torch
