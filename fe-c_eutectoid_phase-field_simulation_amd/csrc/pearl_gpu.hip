@@ -974,6 +974,19 @@ int run_fixed(pearl_ctx* c, const double* dts, int n, double* save = nullptr) {
     h.cg_peak = 0;
     h.phase = PH_START;
     st = write_scalars(c);
+    // Whatever happens below, the context leaves fixed-dt mode: the next
+    // advance must run under the adaptive controller.
+    struct Leave {
+        pearl_ctx* c;
+        ~Leave() {
+            DevScal& h = c->hS[0];
+            h.fixed = 0;
+            h.fixed_dts = nullptr;
+            h.fixed_save = nullptr;
+            h.phase = PH_IDLE;
+            (void)write_scalars(c);
+        }
+    } leave{c};
     if (st) return st;
     long long batch = std::min(n, 64);
     while (true) {
@@ -986,13 +999,7 @@ int run_fixed(pearl_ctx* c, const double* dts, int n, double* save = nullptr) {
     }
     const int code = h.status == ST_OK ? PEARL_OK : status_of(c, 0);
     if (h.cg_peak > 0) c->cg_slots = std::min(24, std::max(3, h.cg_peak + 2));
-    h.fixed = 0;
-    h.fixed_dts = nullptr;
-    h.fixed_save = nullptr;
     h.status = ST_OK;
-    h.phase = PH_IDLE;
-    st = write_scalars(c);
-    if (st) return st;
     c->sc[0].time = h.time;
     c->sc[0].step = h.step;
     return code;

@@ -211,3 +211,35 @@ def test_run_fixed_equals_step_fixed(pkg):
         for x, y in zip(a.get_state()[:3], b.get_state()[:3]):
             assert np.array_equal(x, y)
         assert a.scalars().time == b.scalars().time and a.scalars().step == b.scalars().step
+
+
+def test_fixed_entry_points_reject_bad_arguments_and_leave_fixed_mode(pkg):
+    g, pd, mu_ref, dts, target, *_ = setup()
+    with pkg.Solver(pd, mu_ref) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        for bad in ([], [0.0], [1e-12, float("nan")], [-1e-12]):
+            with pytest.raises(pkg.PearlError):
+                s.run_fixed(bad)
+        with pytest.raises(pkg.PearlError):
+            s.adjoint(dts[:2], target, every=0)
+        with pytest.raises(pkg.PearlError):
+            s.adjoint(dts[:2], target, every=1, w_front=-1.0)
+        # after fixed steps the context advances under the adaptive
+        # controller exactly like a fresh context from the same state
+        s.run_fixed(dts[:3])
+        phi, c, mu, sc = s.get_state()
+        for _ in range(4):
+            s.advance(1e9)
+        a = s.get_state()
+    with pkg.Solver(pd, mu_ref) as f:
+        f.set_state(phi, c, mu, sc)
+        for _ in range(4):
+            f.advance(1e9)
+        b = f.get_state()
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    assert a[3].time == b[3].time and a[3].step == b[3].step
+    with pkg.Solver(dict(pd, phase_only=1), mu_ref) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        with pytest.raises(pkg.PearlError):
+            s.run_fixed(dts[:2])
