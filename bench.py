@@ -110,7 +110,7 @@ def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
                          f"state, serial ({'pearl_serial.cpp advance' if kind == 'reference' else 'oracle'}), "
                          f"{el:.1f} s")
     if kind != "reference" or not ref_tool.mpiexec() or not ref_tool.parallel_program():
-        return serial, serial
+        return serial, serial, None
     nprocs = max(1, min(16, os.cpu_count() or 1))
     while nprocs > 1:
         try:
@@ -120,17 +120,57 @@ def cpu_baseline(path, cfg, phi, c, mu, initial_carbon, seconds, tmpdir):
             nprocs -= 1
     mpi_steps = max(4, int(steps / el * seconds * nprocs * 0.5))
     out = os.path.join(tmpdir, "baseline_mpi.json")
+    work = os.path.join(tmpdir, "mpi")
     try:
-        ref_tool.call("baseline-mpi", path, state, nprocs, mpi_steps, out, timeout=int(seconds * 40 + 300))
+        ref_tool.call("baseline-mpi", path, state, nprocs, mpi_steps, out, work, timeout=int(seconds * 40 + 300))
     except Exception as e:  # noqa: BLE001
         log(f"cpu baseline: pearl_parallel run failed ({e}); reporting the serial figure")
-        return serial, serial
+        return serial, serial, None
     r = json.load(open(out))
     mpi = dict(value=n * r["steps"] / r["seconds"] / 1e6, unit="Mcell-steps/s", cores=nprocs, kind="reference",
                sample=f"{r['steps']} accepted steps of the same {cfg.nx}x{cfg.ny} workload, restarted from the same "
                       f"state, unmodified pearl_parallel under mpiexec -n {nprocs} ({r['px']}x{r['py']} ranks), "
                       f"step-loop wall {r['seconds']:.1f} s from its diagnostics.csv")
-    return mpi, serial
+    return mpi, serial, parity_with_reference_run(path, work, mpi_steps)
+
+
+def parity_with_reference_run(path, work, steps):
+    """The drop-in program pearl_gpu restarted from the same reference
+    checkpoint for the same number of steps as the pearl_parallel run; both
+    final checkpoints read back through the reference reader."""
+    import subprocess
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_tool
+    exe = os.path.join(PKG_DIR, "pearl_gpu")
+    gdir = os.path.join(work, "gpu")
+    r = subprocess.run([exe, path, gdir, "--restart", os.path.join(work, "start.h5"), "--max-steps", str(steps)],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"parity run failed: {r.stderr.strip()[-300:]}")
+        return None
+
+    def status(d):
+        return dict(l.split("=", 1) for l in open(os.path.join(d, "STOPPED.txt")).read().splitlines() if "=" in l)
+
+    import hashlib
+    sr, sg = status(os.path.join(work, "run")), status(gdir)
+    a = ref_tool.read_checkpoint(path, os.path.join(work, "run", "checkpoint.h5"), os.path.join(work, "run"))
+    b = ref_tool.read_checkpoint(path, os.path.join(gdir, "checkpoint.h5"), gdir)
+    s0 = ref_tool.read_checkpoint(path, os.path.join(work, "start.h5"), work)
+    out = {"what": f"pearl_gpu and pearl_parallel (MPI) restarted from the same reference checkpoint, {steps} steps each",
+           "steps": [int(sr["step"]), int(sg["step"])], "rejected": [int(sr["rejected"]), int(sg["rejected"])],
+           "cg_total": [int(sr["CG_total"]), int(sg["CG_total"])],
+           "max_abs_dmu_J_per_mol": float(np.max(np.abs(a[2] - b[2]))),
+           "max_abs_dphi": float(np.max(np.abs(a[0] - b[0]))),
+           "max_abs_dc": float(np.max(np.abs(a[1] - b[1]))),
+           # how far the runs moved from the common start (so equal fields are not a stale read)
+           "moved_from_start_max_abs_dphi": float(np.max(np.abs(a[0] - s0[0]))),
+           "moved_from_start_max_abs_dmu": float(np.max(np.abs(a[2] - s0[2]))),
+           "mu_sha1": [hashlib.sha1(a[2].tobytes()).hexdigest()[:12], hashlib.sha1(b[2].tobytes()).hexdigest()[:12]]}
+    log(f"parity at {a[2].shape[1]}x{a[2].shape[0]}: steps {out['steps']}, rejected {out['rejected']}, CG "
+        f"{out['cg_total']}, max|dmu| {out['max_abs_dmu_J_per_mol']:.2e}, max|dphi| {out['max_abs_dphi']:.2e}")
+    return out
 
 
 def build_ensemble(pkg, count, seed, tmpdir):
@@ -690,8 +730,8 @@ def main():
     s.close()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"], out["cpu_baseline_serial"] = cpu_baseline(path, cfg, phi, c, mu, initial_carbon,
-                                                                       args.cpu_seconds, tmp)
+        out["cpu_baseline"], out["cpu_baseline_serial"], out["parity_vs_reference"] = cpu_baseline(
+            path, cfg, phi, c, mu, initial_carbon, args.cpu_seconds, tmp)
     print(json.dumps(out), flush=True)
 
 

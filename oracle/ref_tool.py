@@ -112,7 +112,8 @@ def main(argv):
         z = np.load(state)
         R = O.Ref(inp)
         R.set_state(z["phi"], z["carbon"], z["mu"], z["scalars"])
-        tmp = tempfile.mkdtemp()
+        tmp = argv[6] if len(argv) > 6 else tempfile.mkdtemp()  # keeps start.h5 and run/ there
+        os.makedirs(tmp, exist_ok=True)
         ckpt = os.path.join(tmp, "start.h5")
         R.write_frame(ckpt)
         ny, nx = z["carbon"].shape
