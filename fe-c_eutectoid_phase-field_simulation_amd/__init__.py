@@ -106,13 +106,67 @@ class SmoothReport(C.Structure):
                 ("final_max_phase_change", C.c_double)]
 
 
+class Fill(C.Structure):
+    """pearl_fill: one FILL operation (filling_phase_at, pearl_serial.cpp:1393-1514)."""
+    _fields_ = [("method", C.c_int), ("reserved", C.c_int), ("p", C.c_double * 16),
+                ("dir", (C.c_double * 2) * 3)]
+
+
+# Method codes of pearl_fill, in the order of the PEARL_FILL_* constants.
+FILL_METHODS = ("FILLCUBE", "FILLCYLINDER", "FILLCENTERBOX2D", "FILLYJUNCTION2D", "FILLYJUNCTIONLAMELLAE2D",
+                "FILLADHEREDPEARLITEARM2D", "FILLPEARLITEELLIPSE2D")
+COMPOSITION_MU, COMPOSITION_MEAN = 0, 1
+
+
+def fill_ops(cfg):
+    """The sharp labelling of `cfg` (a config.Config) as pearl_fill
+    operations: its FILL lines for initialization = microsim, or the two
+    boxes of sharp_smooth (theta in the first theta_columns, alpha in the
+    rest, of the rows whose centre (gy + 0.5) dy lies below seed_height,
+    pearl_serial.cpp:1777-1786).  The directions are evaluated as the
+    reference does, cos(angle*pi/180.0) with pi = 3.14159265358979323846
+    (:1396, :1423-1424, :1440-1441, :1487-1488)."""
+    import math
+    pi = 3.14159265358979323846
+    ops = []
+    if cfg.initialization == "microsim":
+        for method, params in cfg.filling:
+            f = Fill()
+            f.method = FILL_METHODS.index(method)
+            for i, v in enumerate(params):
+                f.p[i] = float(v)
+            if method in ("FILLYJUNCTION2D", "FILLYJUNCTIONLAMELLAE2D"):
+                angles = [(k, params[6 + k]) for k in range(3)]
+            elif method in ("FILLADHEREDPEARLITEARM2D", "FILLPEARLITEELLIPSE2D"):
+                angles = [(0, params[2])]
+            else:
+                angles = []
+            for k, a in angles:
+                ang = float(a) * pi / 180.0
+                f.dir[k][0], f.dir[k][1] = math.cos(ang), math.sin(ang)
+            ops.append(f)
+    elif cfg.initialization == "sharp_smooth":
+        rows = 0
+        while rows < cfg.ny and (rows + 0.5) * cfg.dy < cfg.seed_height:
+            rows += 1
+        for phase, x0, x1 in ((1, 0, cfg.theta_columns - 1), (0, cfg.theta_columns, cfg.nx - 1)):
+            f = Fill()
+            f.method = 0
+            f.p[0], f.p[1], f.p[2], f.p[4], f.p[5] = phase, x0, 0, x1, rows - 1
+            ops.append(f)
+    else:
+        raise ValueError(f"initialization = {cfg.initialization} has no sharp labels")
+    arr = (Fill * max(1, len(ops)))(*ops)
+    return arr, len(ops)
+
+
 EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state", "pearl_get_state",
            "pearl_advance", "pearl_run", "pearl_maybe_shift", "pearl_diagnostics", "pearl_propose_phi",
            "pearl_implicit_diffusion", "pearl_reduced_energy", "pearl_total_carbon", "pearl_valid_state",
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
            "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance",
-           "pearl_run_fixed"]
+           "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition"]
 
 
 def lib():
@@ -136,6 +190,8 @@ def lib():
         L.pearl_maybe_shift.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
         L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
+        L.pearl_fill_phases.argtypes = [vp, C.POINTER(Fill), C.c_int]
+        L.pearl_init_composition.argtypes = [vp, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
         L.pearl_step_fixed.argtypes = [vp, C.c_double]
         L.pearl_set_stop_tolerance.argtypes = [vp, C.c_double]
         L.pearl_run_fixed.argtypes = [vp, C.c_int, _dp]
@@ -327,6 +383,20 @@ class Solver:
         self._check(lib().pearl_smooth(self._h, int(max_steps), int(min_steps), float(energy_rtol),
                                        float(max_phase_change), C.byref(r)))
         return {k: getattr(r, k) for k, _ in SmoothReport._fields_}
+
+    def fill_phases(self, ops, count: int | None = None):
+        """Sharp FILL labels of the current phases (the labelling loop of
+        initialize, pearl_serial.cpp:1764-1795); `ops` from fill_ops."""
+        n = len(ops) if count is None else int(count)
+        self._check(lib().pearl_fill_phases(self._h, ops, n))
+
+    def init_composition(self, mode: int, value: float):
+        """Uniform mu and the carbon of the current phases (initialize
+        :1813-1836).  Returns (mu, initial_carbon, valid)."""
+        mu, carbon, valid = C.c_double(0), C.c_double(0), C.c_int(0)
+        self._check(lib().pearl_init_composition(self._h, int(mode), float(value), C.byref(mu), C.byref(carbon),
+                                                 C.byref(valid)))
+        return mu.value, carbon.value, bool(valid.value)
 
     # single reference functions ------------------------------------------------
     def propose_phi(self, dt: float):

@@ -1176,17 +1176,8 @@ void write_thermo_report(const Config& c, const std::filesystem::path& path) {
 // initialize :1761-1837
 using Observer = std::function<void(const Fields&, const char*, int)>;
 
-SmoothReport smooth_device(Fields& F, const Config& c, int device);
-
-// smooth_device >= 0: the capillarity-only smoothing runs on that GPU
-// (pearl_smooth, blocked summation order) instead of on the host.
-double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const Observer& obs, int smooth_on = -1) {
+void label_cells(Fields& F, const Config& c) {  // the labelling loop :1764-1795
     const int nx = c.nx, ny = c.ny;
-    F.nx = nx;
-    F.ny = ny;
-    F.phi.assign(3 * F.n(), 0.0);
-    F.c.assign(F.n(), 0.0);
-    F.mu.assign(F.n(), 0.0);
     const double width = c.theta_columns * c.dx, L = nx * c.dx, eps = c.eps;
     for (int y = 1; y <= ny; ++y)
         for (int x = 1; x <= nx; ++x) {
@@ -1213,9 +1204,21 @@ double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const 
             }
             F.set(static_cast<std::size_t>(y - 1) * nx + (x - 1), p);
         }
-    if (c.initialization == "sharp_smooth" || c.initialization == "microsim") {
+}
+
+bool sharp_labels(const Config& c) { return c.initialization == "sharp_smooth" || c.initialization == "microsim"; }
+
+double initialize(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const Observer& obs) {
+    const int nx = c.nx, ny = c.ny;
+    F.nx = nx;
+    F.ny = ny;
+    F.phi.assign(3 * F.n(), 0.0);
+    F.c.assign(F.n(), 0.0);
+    F.mu.assign(F.n(), 0.0);
+    label_cells(F, c);
+    if (sharp_labels(c)) {
         if (obs) obs(F, "init_sharp", 0);
-        SmoothReport r = smooth_on >= 0 ? smooth_device(F, c, smooth_on) : smooth(F, c);
+        SmoothReport r = smooth(F, c);
         if (rep) *rep = r;
         if (obs) obs(F, "init_smoothed", r.steps);
     } else if (rep) {
@@ -1254,40 +1257,121 @@ void check(pearl_ctx* g, int st) {
     if (st != PEARL_OK) throw std::runtime_error(g ? pearl_last_error(g) : "pearl_gpu call failed");
 }
 
-// smooth_interface_only (:869-958) on the GPU: the sharp labels go up once,
-// pearl_smooth runs the pseudo-steps, the smoothed phases come back.  The
-// same controls and errors as the host version; the energies are summed in
-// the blocked order, so a marginal accept decision can differ from the
-// host's sequential sums.
-SmoothReport smooth_device(Fields& F, const Config& c, int device) {
-    if (c.smooth_steps < 1 || c.smooth_min_steps < 0 || c.smooth_min_steps > c.smooth_steps || !(c.smooth_energy_rtol > 0) ||
-        !(c.smooth_max_phase_change > 0) || c.smooth_max_phase_change > .2)
+// The sharp labelling of initialize as pearl_fill operations: the FILL
+// lines (microsim), or sharp_smooth's two boxes -- theta in the first
+// theta_columns, alpha in the rest, of the rows whose centre (gy + 0.5) dy
+// lies below seed_height (:1777-1786).  The directions are the reference's
+// expressions (:1423-1424, :1440-1441, :1487-1488).
+std::vector<pearl_fill> fill_ops(const Config& c) {
+    static const char* const kMethods[] = {"FILLCUBE", "FILLCYLINDER", "FILLCENTERBOX2D", "FILLYJUNCTION2D",
+                                           "FILLYJUNCTIONLAMELLAE2D", "FILLADHEREDPEARLITEARM2D",
+                                           "FILLPEARLITEELLIPSE2D"};
+    const double pi = 3.14159265358979323846;
+    std::vector<pearl_fill> ops;
+    if (c.initialization == "microsim") {
+        for (const auto& f : c.fills) {
+            pearl_fill o{};
+            o.method = -1;
+            for (int k = 0; k < 7; ++k)
+                if (f.method == kMethods[k]) o.method = k;
+            if (o.method < 0 || f.p.size() > PEARL_FILL_PARAMS)
+                throw std::invalid_argument("unsupported filling method: " + f.method);
+            std::copy(f.p.begin(), f.p.end(), o.p);
+            auto direction = [&](int k, double degrees) {
+                const double a = degrees * pi / 180.0;
+                o.dir[k][0] = std::cos(a);
+                o.dir[k][1] = std::sin(a);
+            };
+            if (o.method == PEARL_FILL_YJUNCTION2D || o.method == PEARL_FILL_YJUNCTIONLAMELLAE2D)
+                for (int k = 0; k < 3; ++k) direction(k, f.p[6 + k]);
+            else if (o.method == PEARL_FILL_ADHEREDPEARLITEARM2D || o.method == PEARL_FILL_PEARLITEELLIPSE2D)
+                direction(0, f.p[2]);
+            ops.push_back(o);
+        }
+    } else {
+        int rows = 0;
+        while (rows < c.ny && (rows + .5) * c.dy < c.seed_height) ++rows;
+        const int boxes[2][3] = {{1, 0, c.theta_columns - 1}, {0, c.theta_columns, c.nx - 1}};
+        for (const auto& b : boxes) {
+            pearl_fill o{};
+            o.method = PEARL_FILL_CUBE;
+            o.p[0] = b[0];
+            o.p[1] = b[1];
+            o.p[4] = b[2];
+            o.p[5] = rows - 1;
+            ops.push_back(o);
+        }
+    }
+    return ops;
+}
+
+// --gpu-init (no reference counterpart): initialize with its per-cell work
+// on the GPU, on a context of its own.  Sharp labels (pearl_fill_phases) and
+// the capillarity-only smoothing (pearl_smooth) for sharp_smooth and
+// microsim; the diffuse profiles of the other initializations stay on the
+// host.  Then the carbon set-up (pearl_init_composition).  The labels and
+// the per-cell carbon are the reference's bit for bit; the smoothing's
+// energies, a mean_carbon mu and the initial carbon are summed in the
+// blocked order.
+double initialize_device(Fields& F, Meta& s, const Config& c, SmoothReport* rep, const Observer& obs, int device) {
+    if (sharp_labels(c) && (c.smooth_steps < 1 || c.smooth_min_steps < 0 || c.smooth_min_steps > c.smooth_steps ||
+                            !(c.smooth_energy_rtol > 0) || !(c.smooth_max_phase_change > 0) ||
+                            c.smooth_max_phase_change > .2))
         throw std::invalid_argument("invalid sharp-smoothing controls");
-    SmoothReport rep;
-    rep.before = phase_fractions(F);
+    F.nx = c.nx;
+    F.ny = c.ny;
+    F.phi.assign(3 * F.n(), 0.0);
+    F.c.assign(F.n(), 0.0);
+    F.mu.assign(F.n(), 0.0);
     pearl_ctx* g = nullptr;
     check(nullptr, pearl_create(&c.ctl, c.initial_mu, PEARL_FP64, device, 0, &g));
-    pearl_smooth_report r{};
-    int st = PEARL_OK;
-    {
-        const std::vector<double> zero(F.n(), 0.0);  // carbon and mu play no part in the smoothing
+    struct Destroy {
+        pearl_ctx* g;
+        ~Destroy() { pearl_destroy(g); }
+    } guard{g};
+    if (sharp_labels(c)) {
+        const std::vector<pearl_fill> ops = fill_ops(c);
+        check(g, pearl_fill_phases(g, ops.data(), static_cast<int>(ops.size())));
+        check(g, pearl_get_state(g, F.phi.data(), nullptr, nullptr, nullptr));
+        if (obs) obs(F, "init_sharp", 0);
+        SmoothReport r;
+        r.before = phase_fractions(F);
+        pearl_smooth_report sr{};
+        check(g, pearl_smooth(g, c.smooth_steps, c.smooth_min_steps, c.smooth_energy_rtol, c.smooth_max_phase_change,
+                              &sr));
+        check(g, pearl_get_state(g, F.phi.data(), nullptr, nullptr, nullptr));
+        r.steps = sr.steps;
+        r.converged = sr.converged != 0;
+        r.e0 = sr.energy_initial;
+        r.e1 = sr.energy_final;
+        r.rel = sr.final_relative_energy_change;
+        r.change = sr.final_max_phase_change;
+        r.after = phase_fractions(F);
+        if (rep) *rep = r;
+        if (obs) obs(F, "init_smoothed", r.steps);
+    } else {
+        label_cells(F, c);
+        if (rep) {
+            rep->before = phase_fractions(F);
+            rep->after = rep->before;
+            rep->e0 = interface_energy(F, c);
+            rep->e1 = rep->e0;
+            rep->converged = true;
+        }
         pearl_scalars sc{};
-        st = pearl_set_state(g, F.phi.data(), zero.data(), zero.data(), &sc);
+        check(g, pearl_set_state(g, F.phi.data(), F.c.data(), F.mu.data(), &sc));
     }
-    if (st == PEARL_OK)
-        st = pearl_smooth(g, c.smooth_steps, c.smooth_min_steps, c.smooth_energy_rtol, c.smooth_max_phase_change, &r);
-    if (st == PEARL_OK) st = pearl_get_state(g, F.phi.data(), nullptr, nullptr, nullptr);
-    const std::string msg = st == PEARL_OK ? "" : pearl_last_error(g);
-    pearl_destroy(g);
-    if (st != PEARL_OK) throw std::runtime_error(msg);
-    rep.steps = r.steps;
-    rep.converged = r.converged != 0;
-    rep.e0 = r.energy_initial;
-    rep.e1 = r.energy_final;
-    rep.rel = r.final_relative_energy_change;
-    rep.change = r.final_max_phase_change;
-    rep.after = phase_fractions(F);
-    return rep;
+    const bool mean = c.composition_mode == "mean";
+    double mu = 0, carbon = 0;
+    int valid = 0;
+    check(g, pearl_init_composition(g, mean ? PEARL_COMPOSITION_MEAN : PEARL_COMPOSITION_MU,
+                                    mean ? c.initial_carbon : c.initial_mu, &mu, &carbon, &valid));
+    check(g, pearl_get_state(g, nullptr, F.c.data(), F.mu.data(), nullptr));
+    s = Meta{};
+    s.dt = c.dt;
+    s.initial_carbon = carbon;
+    if (!valid) throw std::runtime_error("initial state out of model range");
+    return mu;
 }
 
 pearl_scalars to_scalars(const Meta& m) {
@@ -1380,7 +1464,8 @@ int main(int argc, char** argv) {
             write_initial_phases(fl, config, std::filesystem::path(output) / (std::string(stage) + ".h5"), stage, pseudo);
         };
         const double mu_reference =
-            restart.empty() ? initialize(F, state, config, &rep, writer, gpu_init ? device : -1)
+            restart.empty() ? (gpu_init ? initialize_device(F, state, config, &rep, writer, device)
+                                        : initialize(F, state, config, &rep, writer))
                             : read_checkpoint(F, state, config, restart);
         if (restart.empty()) {
             std::ofstream log(std::filesystem::path(output) / "initialization_report.txt");

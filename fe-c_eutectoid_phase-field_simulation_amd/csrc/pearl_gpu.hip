@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1315,6 +1316,100 @@ int pearl_smooth(pearl_ctx* c, int max_steps, int min_steps, double energy_rtol,
     if (st) return st;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return result;
+}
+
+static_assert(sizeof(FillOp) == sizeof(pearl_fill) && offsetof(FillOp, p) == offsetof(pearl_fill, p) &&
+                  offsetof(FillOp, dir) == offsetof(pearl_fill, dir),
+              "FillOp mirrors pearl_fill");
+static_assert(FILL_CUBE == PEARL_FILL_CUBE && FILL_CYLINDER == PEARL_FILL_CYLINDER &&
+                  FILL_CENTERBOX == PEARL_FILL_CENTERBOX2D && FILL_YJUNCTION == PEARL_FILL_YJUNCTION2D &&
+                  FILL_YJUNCTION_LAMELLAE == PEARL_FILL_YJUNCTIONLAMELLAE2D &&
+                  FILL_ADHERED_ARM == PEARL_FILL_ADHEREDPEARLITEARM2D && FILL_ELLIPSE == PEARL_FILL_PEARLITEELLIPSE2D,
+              "method codes of the C ABI");
+
+int pearl_fill_phases(pearl_ctx* c, const pearl_fill* ops, int count) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if (count < 0 || (count > 0 && !ops)) return fail(c, PEARL_ERR_ARGUMENT, "invalid FILL operation list");
+    // What the kernel relies on: a known method, phase IDs 0..2 where a
+    // label is taken from a parameter, finite parameters and directions, and
+    // at least one pair on a pearlite arm.  The rest is read_config's.
+    for (int k = 0; k < count; ++k) {
+        const pearl_fill& f = ops[k];
+        bool ok = f.method >= 0 && f.method < FILL_METHODS;
+        for (double v : f.p) ok = ok && std::isfinite(v);
+        for (const auto& u : f.dir) ok = ok && std::isfinite(u[0]) && std::isfinite(u[1]);
+        auto phase = [&](int i) { ok = ok && (f.p[i] == 0.0 || f.p[i] == 1.0 || f.p[i] == 2.0); };
+        if (ok) {
+            switch (f.method) {
+                case FILL_CUBE:
+                case FILL_CYLINDER: phase(0); break;
+                case FILL_CENTERBOX: phase(4), phase(5); break;
+                case FILL_YJUNCTION: phase(11), phase(12); break;
+                case FILL_YJUNCTION_LAMELLAE: phase(13), phase(14), phase(15); break;
+                default: phase(9), phase(10); break;  // pearlite arm, ellipse
+            }
+            if (f.method == FILL_ADHERED_ARM) ok = ok && f.p[4] >= 1.0 && f.p[4] <= 2147483647.0;
+        }
+        if (!ok) return fail(c, PEARL_ERR_ARGUMENT, "invalid FILL operation " + std::to_string(k));
+    }
+    st = read_scalars(c);
+    if (st) return st;
+    const KP& P = c->kp[0];
+    FillOp* d_ops = nullptr;
+    if (count > 0) {
+        HIP_CHECK(c, hipMalloc(&d_ops, static_cast<size_t>(count) * sizeof(FillOp)));
+        const hipError_t e =
+            hipMemcpyAsync(d_ops, ops, static_cast<size_t>(count) * sizeof(FillOp), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(d_ops);
+            return hip_fail(c, e, "hipMemcpyAsync(fill operations)");
+        }
+    }
+    PEARL_LAUNCH(c, k_fill, dim3(static_cast<unsigned>(P.nb)), c->B, 0, d_ops, count);
+    hipError_t e = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = e2;
+    if (d_ops) (void)hipFree(d_ops);
+    if (e != hipSuccess) return hip_fail(c, e, "k_fill");
+    // the energy and mass of the previous phases no longer hold
+    DevScal& h = c->hS[0];
+    h.e0_valid = 0;
+    h.m0_valid = 0;
+    return write_scalars(c);
+}
+
+int pearl_init_composition(pearl_ctx* c, int mode, double value, double* mu_out, double* carbon_out,
+                           int* valid_out) {
+    int st = ctx_check(c, 0);
+    if (st) return st;
+    if ((mode != PEARL_COMPOSITION_MU && mode != PEARL_COMPOSITION_MEAN) || !std::isfinite(value) || !mu_out ||
+        !carbon_out || !valid_out)
+        return fail(c, PEARL_ERR_ARGUMENT, "invalid composition set-up");
+    st = read_scalars(c);
+    if (st) return st;
+    const KP& P = c->kp[0];
+    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+    DevScal& h = c->hS[0];
+    double mu = value;
+    if (mode == PEARL_COMPOSITION_MEAN) {  // :1814-1822
+        PEARL_LAUNCH(c, k_init_sums, grid, c->B, 0);
+        hipLaunchKernelGGL(k_init_sums_fin, one, tb, 0, c->stream, c->B, 0);
+        st = read_scalars(c);
+        if (st) return st;
+        if (h.in_bad > 0) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+        mu = (static_cast<double>(P.nx * P.ny) * value - h.in_c0) / h.in_s;
+    }
+    PEARL_LAUNCH(c, k_init_carbon, grid, c->B, 0, mu);
+    double valid = 0, mass = 0;
+    st = state_direct(c, 0, true, &valid, &mass, nullptr);
+    if (st) return st;
+    *mu_out = mu;
+    *carbon_out = mass;
+    *valid_out = valid > 0.5 ? 1 : 0;
+    h.e0_valid = 0;
+    h.m0_valid = 0;
+    return write_scalars(c);
 }
 
 int pearl_reduced_energy(pearl_ctx* c, double* energy) {

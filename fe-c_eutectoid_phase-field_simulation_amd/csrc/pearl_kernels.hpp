@@ -104,6 +104,9 @@ struct DevScal {
     double dmin_c, dmax_c, dmin_mu, dmax_mu, dsumerr, dmass, denergy;
     // interface smoothing: max |force|, max phase change, energy, failure flag
     double sm_fmax, sm_change, sm_energy, sm_bad;
+    // initialization: sums of the carbon intercept and susceptibility, and
+    // an invalid-mixture flag
+    double in_c0, in_s, in_bad;
     // StepInfo of the last accepted step
     double i_dt, i_change, i_residual, i_closure, i_energy_change;
     int i_cg, i_retries;
@@ -1716,5 +1719,209 @@ __global__ __launch_bounds__(TPB) void k_iface_energy_fin(Bufs B, int d) {
 
 // Clears error bits (a kernel, not a memset, so it is safe in graphs).
 __global__ void k_clear_err(Bufs B, int d) { B.S[d].err = 0; }
+
+// ---------------------------------------------------------------------------
+// Initialization (initialize :1761-1837): the sharp labels of the FILL
+// recipe and the carbon / chemical-potential set-up.
+// ---------------------------------------------------------------------------
+// One FILL operation, laid out as the C ABI's pearl_fill: the parameters in
+// input order and the cosine and sine of each direction the operation uses
+// (Y-junction arm k: dir[k]; pearlite arm and ellipse: dir[0]).  The host
+// evaluates them with the reference's expressions, so the labels do not
+// depend on the device's cos/sin.
+struct FillOp {
+    int method, pad;
+    double p[16];
+    double dir[3][2];
+};
+enum { FILL_CUBE = 0, FILL_CYLINDER, FILL_CENTERBOX, FILL_YJUNCTION, FILL_YJUNCTION_LAMELLAE, FILL_ADHERED_ARM,
+       FILL_ELLIPSE, FILL_METHODS };
+
+// filling_centered_lamellar_phase :1378-1389: cementite in the middle of
+// every period.
+__host__ __device__ inline int centred_lamella(double along, double period, double fraction, int ferrite,
+                                               int cementite) {
+    double pos = fmod(along, period);
+    if (pos < 0.0) pos += period;
+    const double width = fraction * period;
+    return (pos >= 0.5 * (period - width) && pos < 0.5 * (period + width)) ? cementite : ferrite;
+}
+
+// Rotated coordinates of cell offset (ex, ey) along direction (ux, uy).
+__host__ __device__ inline void along_across(double ex, double ey, double ux, double uy, double& along,
+                                             double& across) {
+    along = ex * ux + ey * uy;
+    across = -ex * uy + ey * ux;
+}
+
+// Label of cell (gx, gy) after operation f, given its label `cur` before
+// (filling_phase_at :1393-1510).  Whole-map recipes (centre box, Y
+// junctions) set every cell; overlays change only the cells of their mask.
+// Host-callable too, so that the labelling can be checked on the CPU.
+__host__ __device__ inline int fill_label(const FillOp& f, int gx, int gy, int cur) {
+    const double* p = f.p;
+    constexpr double pi = 3.14159265358979323846;  // the literal of :1396
+    switch (f.method) {
+        case FILL_CUBE: {  // inclusive box; a gamma cube changes nothing (:1399-1405)
+            const int s = static_cast<int>(p[0]);
+            return (s != 2 && gx >= p[1] && gx <= p[4] && gy >= p[2] && gy <= p[5]) ? s : cur;
+        }
+        case FILL_CYLINDER: {  // disc of radius p[5] about (p[1], p[2]) (:1406-1410)
+            const double ex = gx - p[1], ey = gy - p[2];
+            const int s = static_cast<int>(p[0]);
+            return (s != 2 && ex * ex + ey * ey <= p[5] * p[5]) ? s : cur;
+        }
+        case FILL_CENTERBOX: {  // half-open box, inside p[4], outside p[5] (:1411-1413)
+            const bool in = gx >= p[0] && gx < p[1] && gy >= p[2] && gy < p[3];
+            return static_cast<int>(in ? p[4] : p[5]);
+        }
+        case FILL_YJUNCTION:
+        case FILL_YJUNCTION_LAMELLAE: {  // three arms inside a window (:1414-1438)
+            const bool lam = f.method == FILL_YJUNCTION_LAMELLAE;
+            const int ferrite = static_cast<int>(p[lam ? 13 : 11]);
+            const int background = static_cast<int>(p[lam ? 15 : 12]);
+            if (!(gx >= p[0] && gx < p[1] && gy >= p[2] && gy < p[3])) return background;
+            const double ex = gx - p[4], ey = gy - p[5];
+            for (int arm = 0; arm < 3; ++arm) {  // the first arm that holds the cell decides
+                double along, across;
+                along_across(ex, ey, f.dir[arm][0], f.dir[arm][1], along, across);
+                if (along >= 0.0 && along <= p[9] && fabs(across) <= p[10]) {
+                    if (!lam) return ferrite;
+                    const double period = p[11] + p[12];
+                    double pos = fmod(along, period);
+                    if (pos < 0.0) pos += period;
+                    return pos < p[11] ? ferrite : static_cast<int>(p[14]);
+                }
+            }
+            return background;
+        }
+        case FILL_ADHERED_ARM: {  // lamellar arm with a sine-shaped outline (:1439-1485)
+            double along, across;
+            along_across(gx - p[0], gy - p[1], f.dir[0][0], f.dir[0][1], along, across);
+            const double len = p[3];
+            const int pairs = static_cast<int>(p[4]);
+            const double period = len / static_cast<double>(pairs);
+            const double fw = (1.0 - p[5]) * period, cw = p[5] * period;
+            const double peak = p[6], contact = p[7], bridge = p[8];
+            const int ferrite = static_cast<int>(p[9]), cementite = static_cast<int>(p[10]);
+            int s = -1;
+            if (along >= 0.0 && along <= len) {
+                int pid = along >= len ? pairs - 1 : static_cast<int>(floor(along / period));
+                pid = pid < pairs - 1 ? pid : pairs - 1;  // std::max(0, std::min(pairs - 1, pid))
+                pid = pid < 0 ? 0 : pid;
+                const double start = static_cast<double>(pid) * period;
+                const bool first = along - start < fw;  // ferrite, then cementite within a pair
+                const int seg = first ? 2 * pid : 2 * pid + 1;
+                const double seg_start = first ? start : start + fw;
+                s = first ? ferrite : cementite;
+                double t = (along - seg_start) / (first ? fw : cw);
+                if (t < 0.0) t = 0.0;
+                else if (t > 1.0) t = 1.0;
+                const double h0 = seg == 0 ? 0.0 : contact;
+                const double h1 = seg == 2 * pairs - 1 ? 0.0 : contact;
+                // sin: the device's (the reference calls std::sin); a label can
+                // differ only where |across| is within an ulp of the outline.
+                double half = (1.0 - t) * h0 + t * h1 + (peak - 0.5 * (h0 + h1)) * sin(pi * t);
+                if (half > peak) half = peak;
+                if (fabs(across) > half) s = -1;
+            }
+            if (bridge > 0.0) {
+                const double b2 = bridge * bridge;
+                if (along * along + across * across <= b2) s = ferrite;
+                if ((along - len) * (along - len) + across * across <= b2) s = cementite;
+            }
+            return s >= 0 ? s : cur;
+        }
+        case FILL_ELLIPSE: {  // lamellar ellipse with optional end bridges (:1486-1508)
+            double along, across;
+            along_across(gx - p[0], gy - p[1], f.dir[0][0], f.dir[0][1], along, across);
+            const double start = p[3], period = p[5], length = p[4] * period;
+            const double semi_major = 0.5 * length, semi_minor = p[6] * semi_major;
+            const double u = (along - (start + semi_major)) / semi_major, v = across / semi_minor;
+            const double bridge = p[8];
+            const int ferrite = static_cast<int>(p[9]), cementite = static_cast<int>(p[10]);
+            int s = -1;
+            if (u * u + v * v <= 1.0) s = centred_lamella(along - start, period, p[7], ferrite, cementite);
+            if (bridge > 0.0) {
+                const double a0 = along - start, a1 = along - (start + length);
+                const double b2 = bridge * bridge;
+                if (a0 * a0 + across * across <= b2 || a1 * a1 + across * across <= b2) s = ferrite;
+            }
+            return s >= 0 ? s : cur;
+        }
+        default: return cur;
+    }
+}
+
+// Sharp labels of the current phases: every cell starts as gamma and the
+// operations apply in order.  The operation list is read with uniform
+// (scalar) loads: every lane walks the same list.
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_fill(Bufs B, int d, const FillOp* __restrict__ ops, int count) {
+    const KP P = B.kp[d];
+    const int lb = blockIdx.x;
+    double* __restrict__ phi = B.phi[B.S[d].cur] + 3 * P.off;
+    const int n = P.n;
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int y) {
+        int s = 2;
+        for (int k = 0; k < count; ++k) s = fill_label(ops[k], col.x, y, s);
+        phi[i] = s == 0 ? 1.0 : 0.0;
+        phi[n + i] = s == 1 ? 1.0 : 0.0;
+        phi[2 * n + i] = s == 2 ? 1.0 : 0.0;
+    });
+}
+
+// Sums over the cells of the carbon intercept and the susceptibility of
+// the current phases (the mean_carbon branch of initialize :1814-1822),
+// blocked order; q2 flags a cell whose mixture the reference rejects.
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_init_sums(Bufs B, int d) {
+    const KP P = B.kp[d];
+    const int lb = blockIdx.x;
+    const double* __restrict__ phi = B.phi[B.S[d].cur] + 3 * P.off;
+    double v[3] = {0.0, 0.0, 0.0};
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int) {
+        double p[3];
+        load_phi(phi, P.n, i, p);
+        Mix m;
+        if (!mixture(P, p, m)) v[2] = 1.0;
+        v[0] += m.c0;
+        v[1] += m.s;
+    });
+    block_partial<3, (1u << 4)>(v, B.part + P.poff, P.nb, lb);
+}
+
+__global__ __launch_bounds__(TPB) void k_init_sums_fin(Bufs B, int d) {
+    const KP P = B.kp[d];
+    DevScal* S = B.S + d;
+    double v[3];
+    if (!final_reduce<3, (1u << 4)>(v, B.part + P.poff, P.nb)) return;
+    S->in_c0 = v[0];
+    S->in_s = v[1];
+    S->in_bad = v[2];
+}
+
+// Uniform chemical potential mu and the carbon of the current phases at it
+// (initialize :1823-1826): c = composition(mu, mixture(phi)).
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_init_carbon(Bufs B, int d, double mu) {
+    const KP P = B.kp[d];
+    const int lb = blockIdx.x;
+    const int cur = B.S[d].cur;
+    const double* __restrict__ phi = B.phi[cur] + 3 * P.off;
+    double* __restrict__ c = B.c[cur] + P.off;
+    double* __restrict__ m_out = B.mu[cur] + P.off;
+    const Col col = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, col, [&](const int i, const int) {
+        double p[3];
+        load_phi(phi, P.n, i, p);
+        Mix m;
+        mixture(P, p, m);  // an invalid mixture is reported by the state check that follows
+        c[i] = composition(mu, m);
+        m_out[i] = mu;
+    });
+}
 
 }  // namespace pearl_gpu

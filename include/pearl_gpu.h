@@ -104,6 +104,33 @@ typedef struct {
     double c_min, c_max, mu_min, mu_max, phi_sum_error, mass, energy;
 } pearl_diag;
 
+/* One FILL operation of initialization = microsim (filling_phase_at
+ * :1393-1514).  method: PEARL_FILL_*.  p: the operation's parameters in
+ * input order, as read_config parsed and validated them
+ * (parse_filling_operation :1209, validate_filling_operations :1251; unused entries
+ * 0).  dir: cos and sin of each direction the operation uses, evaluated by
+ * the caller with the reference's expression, cos(angle*pi/180.0) with the
+ * literal pi = 3.14159265358979323846 (:1396): Y-junction arm k (angle
+ * p[6+k]) in dir[k], the pearlite arm's and the ellipse's angle p[2] in
+ * dir[0]; unused entries 0. */
+#define PEARL_FILL_CUBE 0                /* FILLCUBE                  :1399 */
+#define PEARL_FILL_CYLINDER 1            /* FILLCYLINDER              :1406 */
+#define PEARL_FILL_CENTERBOX2D 2         /* FILLCENTERBOX2D           :1411 */
+#define PEARL_FILL_YJUNCTION2D 3         /* FILLYJUNCTION2D           :1414 */
+#define PEARL_FILL_YJUNCTIONLAMELLAE2D 4 /* FILLYJUNCTIONLAMELLAE2D   :1415 */
+#define PEARL_FILL_ADHEREDPEARLITEARM2D 5 /* FILLADHEREDPEARLITEARM2D :1439 */
+#define PEARL_FILL_PEARLITEELLIPSE2D 6   /* FILLPEARLITEELLIPSE2D     :1486 */
+#define PEARL_FILL_PARAMS 16
+typedef struct {
+    int method, reserved;
+    double p[PEARL_FILL_PARAMS];
+    double dir[3][2];
+} pearl_fill;
+
+/* Composition modes of pearl_init_composition (composition_mode, :1725-1740). */
+#define PEARL_COMPOSITION_MU 0   /* initial_mu or gamma_carbon: mu given */
+#define PEARL_COMPOSITION_MEAN 1 /* mean_carbon: mu from the mean carbon */
+
 /* Result of the capillarity-only smoothing (SmoothReport :843-852, without
  * the phase fractions). */
 typedef struct {
@@ -182,6 +209,33 @@ int pearl_valid_state(pearl_ctx* ctx, int* valid);
  * controls"), PEARL_ERR_SMOOTHING, PEARL_ERR_PROJECTION. */
 int pearl_smooth(pearl_ctx* ctx, int max_steps, int min_steps, double energy_rtol, double max_phase_change,
                  pearl_smooth_report* report);
+
+/* Sharp phase labels of a FILL recipe on the device (replaces the labelling
+ * loop of initialize, :1764-1795, for initialization = microsim, which calls
+ * filling_phase_at :1393-1514 per cell; sharp_smooth's labels are two
+ * FILLCUBE operations).  Every cell starts as gamma and the `count`
+ * operations apply in order; the current phases become one-hot.  Carbon and
+ * mu are left as they are.  Labels equal the reference's bit for bit; the one
+ * per-cell transcendental, the sine of the pearlite arm's outline (:1474),
+ * is the device's, so a cell whose |across| lies within an ulp of that
+ * outline could differ.  Errors: PEARL_ERR_ARGUMENT ("invalid FILL
+ * operation k": unknown method, a phase ID outside 0..2, a non-finite
+ * parameter, or a pair count below 1). */
+int pearl_fill_phases(pearl_ctx* ctx, const pearl_fill* ops, int count);
+
+/* Carbon and chemical potential of the current phases (replaces the end of
+ * initialize, :1813-1836).  mode PEARL_COMPOSITION_MU: mu = value.  mode
+ * PEARL_COMPOSITION_MEAN: value is the mean carbon and
+ * mu = (nx*ny*value - sum intercept) / sum susceptibility (:1814-1822),
+ * summed in the blocked order.  Then mu is uniform and c = composition(mu,
+ * mixture(phi)) in every cell.  Returns mu (initialize's return value, the
+ * run's mu_reference), the total carbon (total_carbon :784, blocked order;
+ * State::initial_carbon) and valid_state (:1096-1116) of the result, which
+ * initialize turns into "initial state out of model range".  The state
+ * scalars are left as they are.  Errors: PEARL_ERR_ARGUMENT,
+ * PEARL_ERR_MIXTURE ("Invalid mixture coefficients"). */
+int pearl_init_composition(pearl_ctx* ctx, int mode, double value, double* mu, double* initial_carbon,
+                           int* valid);
 
 /* One step of size dt that is always taken: propose_phi (:961-1002), then
  * implicit_diffusion (:1010-1094) on the trial phases, then the trial state

@@ -270,6 +270,7 @@ def ref_lib():
         L.ref_params.argtypes = [vp, _dp]
         L.ref_signature.argtypes = [vp, C.c_char_p, C.c_int]
         L.ref_initialize.argtypes = [vp]
+        L.ref_sharp_labels.argtypes = [vp, _dp]
         L.ref_get_state.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.ref_set_state.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.ref_set_controls.argtypes = [vp, _dp]
@@ -324,6 +325,14 @@ class Ref:
 
     def initialize(self):
         self._chk(ref_lib().ref_initialize(self.h))
+
+    def sharp_labels(self):
+        """phi[3, ny, nx] of the reference's initialize at its init_sharp
+        observer call (before smoothing); the handle must be reopened
+        afterwards."""
+        phi = np.empty((3, self.ny, self.nx))
+        self._chk(ref_lib().ref_sharp_labels(self.h, _ptr(phi)))
+        return phi
 
     def get_state(self):
         n = self.nx * self.ny

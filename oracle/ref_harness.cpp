@@ -8,7 +8,7 @@
 // The reference's main() is renamed at compile time (-Dmain=...), so this
 // harness is a plain shared library.  Reference functions used:
 //   read_config        pearl_serial.cpp:1600
-//   initialize         pearl_serial.cpp:1761
+//   initialize         pearl_serial.cpp:1761 (also up to its init_sharp observer call)
 //   propose_phi        pearl_serial.cpp:961
 //   implicit_diffusion pearl_serial.cpp:1010
 //   valid_state        pearl_serial.cpp:1096
@@ -159,6 +159,34 @@ int ref_initialize(void* hp) {
         return -1;
     }
     return 0;
+}
+
+// Sharp labels of the reference's initialize (:1764-1797): the phases its
+// observer receives at "init_sharp", before the smoothing.  The observer
+// leaves initialize by throwing a private type, so the smoothing and the
+// carbon set-up do not run (the handle's state is then partly initialized;
+// reopen before other use).  Returns -2 for an initialization without
+// sharp labels (lamella, flat, circle).
+namespace {
+struct SharpCaptured {};
+}  // namespace
+int ref_sharp_labels(void* hp, double* phi3) {
+    auto* h = static_cast<RefHandle*>(hp);
+    bool got = false;
+    try {
+        pearl::initialize(*h->dom, *h->st, h->cfg, nullptr,
+                          [&](const pearl::Domain& d, const pearl::State& s, const char* stage, int) {
+                              if (std::string(stage) != "init_sharp") return;
+                              get_phi(d, s.phi, phi3);
+                              got = true;
+                              throw SharpCaptured{};
+                          });
+    } catch (const SharpCaptured&) {
+    } catch (const std::exception& e) {
+        set_err(h, e);
+        return -1;
+    }
+    return got ? 0 : -2;
 }
 
 // Scalars: [0] time [1] dt [2] shift_distance [3] initial_carbon

@@ -44,7 +44,7 @@ def test_binding_structs_match_the_header(tmp_path):
     assert gcc, "gcc is part of this image"
     structs = {"pearl_params": pkg.Params, "pearl_scalars": pkg.Scalars, "pearl_step_info": pkg.StepInfo,
                "pearl_diffusion_info": pkg.DiffusionInfo, "pearl_diag": pkg.Diag,
-               "pearl_smooth_report": pkg.SmoothReport}
+               "pearl_smooth_report": pkg.SmoothReport, "pearl_fill": pkg.Fill}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pearl_gpu.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
@@ -61,3 +61,11 @@ def test_binding_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+    # method codes of pearl_fill in the binding's order
+    codes = subprocess.run([gcc, "-std=c99", "-E", "-dM", "-I", os.path.join(ROOT, "include"),
+                            os.path.join(ROOT, "include", "pearl_gpu.h")], capture_output=True, text=True,
+                           check=True).stdout
+    defined = {l.split()[1]: l.split()[2] for l in codes.splitlines() if l.startswith("#define PEARL_FILL_")}
+    for i, name in enumerate(pkg.FILL_METHODS):
+        assert defined["PEARL_FILL_" + name[4:]] == str(i), name
+    assert defined["PEARL_FILL_PARAMS"] == str(len(pkg.Fill().p))

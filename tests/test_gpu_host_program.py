@@ -158,3 +158,35 @@ def test_gpu_init_smoothing_matches_host(tmp_path):
         assert ra[k] == rb[k], k
     for k in ["smooth_energy_initial", "smooth_energy_final"]:
         assert abs(float(ra[k]) - float(rb[k])) <= 1e-12 * abs(float(ra[k])), k
+
+
+def test_gpu_init_fill_recipe_matches_reference(tmp_path):
+    """--gpu-init on a FILL recipe with every method (tests/fill_recipes.py):
+    the sharp labels the program writes (init_sharp.h5) are the reference
+    program's bit for bit (pearl_fill_phases), and so are the smoothed
+    phases and the initialized state (frame 0: pearl_smooth,
+    pearl_init_composition), as every smoothing decision agrees on this
+    input."""
+    import importlib
+
+    import fill_recipes
+    load_pkg()
+    inputs = importlib.import_module("pearl_amd.inputs")
+    tmp = str(tmp_path)
+    inp = inputs.write_input(os.path.join(tmp, "recipe.in"), 300, 170, 3e-10, 1.5e-10,
+                             fill_recipes.recipe(300, 170, 0, "FILLCENTERBOX2D", overlays=8),
+                             overrides={"smooth_steps": 40, "smooth_min_steps": 40})
+    say("gpu-init recipe: reference run")
+    rc, _, err = run(REF, [inp, os.path.join(tmp, "ref"), "--max-steps", "2"])
+    assert rc == 0, err
+    say("gpu-init recipe: gpu run")
+    rc, _, err = run(EXE, [inp, os.path.join(tmp, "gpu"), "--max-steps", "2", "--gpu-init"])
+    assert rc == 0, err
+    for f in ["init_sharp.h5", "init_smoothed.h5"]:
+        r = subprocess.run(["/opt/conda/bin/h5diff", os.path.join(tmp, "ref", f), os.path.join(tmp, "gpu", f)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (f, r.stdout[-2000:])
+    a = load_checkpoint(inp, os.path.join(tmp, "ref", "frame_000000.h5"))
+    b = load_checkpoint(inp, os.path.join(tmp, "gpu", "frame_000000.h5"))
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
