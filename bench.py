@@ -5,7 +5,8 @@ Workload (production FP64 domain): 4096 x 2048 cells, the default 10 K
 phase-diagram input's physics, a colony of 32 ferrite/cementite lamella
 pairs (spacing +/-5 % from a fixed seed, 1:7 cementite:ferrite, colony 256
 rows high) over eutectoid-composition austenite, adaptive dt, implicit
-diffusion, moving window checked at output times.  A "step" is one accepted
+diffusion, moving window checked at output times, 10,000 accepted steps
+timed after 5 warm-up steps (the defaults).  A "step" is one accepted
 step of the host program's loop (all its attempts, PCG iterations and
 acceptance reductions): pearl_run between output times, with the reference
 loop's output tolerance, then the moving-window check at each output time.
@@ -16,7 +17,7 @@ Prints one JSON line:
   roofline  the dominant kernel's algorithmic HBM bytes per launch / its
             average launch time, against 8 TB/s.  The launch times come from
             HIP events on the library's stream around each cell kernel, over
-            a second region of the same number of steps right after the
+            a second region of --kernel-steps steps (500) right after the
             timed one (the events add graph nodes and slow the whole loop by
             ~15 %, so the timed region runs without them); `traffic` is the
             kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE and
@@ -516,7 +517,8 @@ def run_adjoint(args, pkg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=10000,
+                    help="accepted steps in the timed region (the production workload is 10,000)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--ny", type=int, default=2048)
@@ -525,6 +527,8 @@ def main():
     ap.add_argument("--undercooling", type=float, default=10.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=500,
+                    help="accepted steps of the per-kernel event region after the timed one")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the second region with per-kernel events (no roofline)")
     ap.add_argument("--workload", choices=["production", "ensemble", "mixed-accuracy", "adjoint", "smoke"],
@@ -663,21 +667,22 @@ def main():
     d = s.diagnostics()
     log(f"timed {args.steps} steps in {wall:.3f} s; t = {sc.time:.4e}, rejected attempts "
         f"{c1['rejected_attempts'] - c0['rejected_attempts']}, CG {c1['cg_iterations'] - c0['cg_iterations']}")
-    # Kernel region: the next --steps steps with HIP events around each cell
-    # kernel on the library's stream; the per-kernel averages for the roofline.
+    # Kernel region: the next --kernel-steps steps with HIP events around each
+    # cell kernel on the library's stream; the per-kernel averages for the
+    # roofline.
     stats, wall_ev = {}, None
     if not args.no_profile:
         s.profile(True)
         if have_torch:
             torch.cuda.synchronize()
         t1 = time.perf_counter()
-        steps(args.steps)
+        steps(args.kernel_steps)
         if have_torch:
             torch.cuda.synchronize()
         wall_ev = time.perf_counter() - t1
         stats = s.kernel_stats()
         s.profile(False)
-        log(f"kernel region: {args.steps} more steps with per-kernel events in {wall_ev:.3f} s")
+        log(f"kernel region: {args.kernel_steps} more steps with per-kernel events in {wall_ev:.3f} s")
         for name, v in stats.items():
             if v["launches"]:
                 log(f"  {name:16s} {v['launches']:6d} launches {v['ms']:9.3f} ms  "
@@ -693,7 +698,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": dom[1]["bytes"] / dom[1]["launches"],
                 "avg_launch_ms": dom[1]["ms"] / dom[1]["launches"],
-                "region": f"the {args.steps} steps after the timed region, HIP events around each cell kernel",
+                "region": f"the {args.kernel_steps} steps after the timed region, HIP events around each cell kernel",
                 "kernels": {k: {"achieved_GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
                                 "avg_launch_ms": round(v["ms"] / v["launches"], 5), "launches": v["launches"]}
                             for k, v in active.items() if v["bytes"] > 0}}
@@ -721,7 +726,7 @@ def main():
         "rejected_attempts": c1["rejected_attempts"] - c0["rejected_attempts"],
         "cg_iterations": c1["cg_iterations"] - c0["cg_iterations"],
         "loop_algorithmic_GBps": round(total_bytes / wall / 1e9, 1),
-        "with_kernel_events_Mcell_steps_per_s": round(nx * ny * args.steps / wall_ev / 1e6, 3) if wall_ev else None,
+        "with_kernel_events_Mcell_steps_per_s": round(nx * ny * args.kernel_steps / wall_ev / 1e6, 3) if wall_ev else None,
         "roofline": roof,
         "state_check": {"time": sc.time, "step": sc.step, "phi_sum_error": d["phi_sum_error"],
                         "mass_ledger": d["mass"] + sc.carbon_removed - sc.carbon_added - sc.initial_carbon,
