@@ -204,13 +204,15 @@ def build_ensemble(pkg, count, seed, tmpdir):
 
 
 def run_mixed_accuracy(args, pkg):
-    """FP32-mixed form against the FP64 plain form on the same inputs, in
-    lockstep: each step the FP64 context advances, then the mixed context
-    takes a step of exactly the accepted dt (so both follow one dt sequence
-    and the difference is precision alone).  Every --acc-every steps the
-    fields are compared: max |dmu| in J/mol and max |dphi| per phase
-    fraction.  Afterwards both forms run free from their states for the
-    timed throughput comparison."""
+    """FP32-mixed form against the FP64 plain form on the same inputs.
+    --acc-mode lockstep: each step the FP64 context advances, then the mixed
+    context takes a step of exactly the accepted dt (so both follow one dt
+    sequence and the difference is precision alone); the fields are
+    compared every --acc-every steps.  --acc-mode free: each form runs its
+    own adaptive dt and the fields are compared at matched physical times,
+    about every --acc-every FP64 steps.  Compared: max |dmu| in J/mol and
+    max |dphi| per phase fraction.  Afterwards both forms run free from
+    their states for --steps steps, the timed throughput comparison."""
     import numpy as np
     try:
         import torch
@@ -222,8 +224,8 @@ def run_mixed_accuracy(args, pkg):
     uc = 20.0 if args.undercooling == 10.0 else args.undercooling
     tmp = tempfile.mkdtemp()
     path, cfg, phi, c, mu = build_workload(pkg, nx, ny, pairs, ny // 8, args.seed, uc, tmp)
-    log(f"mixed accuracy: {nx}x{ny}, {pairs} pairs, {uc} K, {args.acc_steps} lockstep steps, compared every "
-        f"{args.acc_every}")
+    log(f"mixed accuracy ({args.acc_mode}): {nx}x{ny}, {pairs} pairs, {uc} K, {args.acc_steps} FP64 steps, "
+        f"compared every {args.acc_every}")
     a = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=pkg.FP64)
     if args.acc_other == "mixed":
         b = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread,
@@ -234,7 +236,37 @@ def run_mixed_accuracy(args, pkg):
         s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
     m0 = a.total_carbon()
     rows, broken = [], None
-    for k in range(1, args.acc_steps + 1):
+    if args.acc_mode == "free":
+        # Each form runs its own adaptive dt; they are compared at matched
+        # physical times t_k = k T, where T is the time the FP64 form reaches
+        # in its first --acc-every steps.  pearl_run clips the last step of
+        # each interval to t_k, so both states are taken exactly there.
+        big = 10 ** 8
+        T = None
+        for k in range(1, max(1, args.acc_steps // args.acc_every) + 1):
+            sa0, sb0 = a.scalars().step, b.scalars().step
+            if T is None:
+                a.run(1.0, args.acc_every)
+                T = a.scalars().time
+                t_k = T
+            else:
+                t_k = k * T
+                a.run(t_k, big)
+            b.run(t_k, big)
+            ta, tb = a.scalars().time, b.scalars().time
+            pa, ca, ma, _ = a.get_state()
+            pb, cb, mb, _ = b.get_state()
+            row = {"interval": k, "time": ta, "time_other": tb,
+                   "steps": [a.scalars().step, b.scalars().step],
+                   "interval_steps": [a.scalars().step - sa0, b.scalars().step - sb0],
+                   "max_dmu_J_per_mol": float(np.max(np.abs(ma - mb))),
+                   "max_dphi": [float(np.max(np.abs(pa[i] - pb[i]))) for i in range(3)],
+                   "max_dc": float(np.max(np.abs(ca - cb))),
+                   "mass_rel_mixed": abs(b.total_carbon() - m0) / abs(m0)}
+            rows.append(row)
+            log(f"  t={ta:.4e} (other {tb:.4e}) steps {row['steps']}: max|dmu| {row['max_dmu_J_per_mol']:.3e} "
+                f"J/mol, max|dphi| {row['max_dphi']}")
+    for k in range(1, args.acc_steps + 1 if args.acc_mode == "lockstep" else 1):
         info = a.advance(1.0)
         sc = b.scalars()
         sc.dt = info.accepted_dt
@@ -279,10 +311,13 @@ def run_mixed_accuracy(args, pkg):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": f"{'FP32-mixed form' if args.acc_other == 'mixed' else 'FP64 with 8-row tiles'} "
                                f"against FP64 on {nx}x{ny}, {pairs} pairs +/-5% (seed {args.seed}), "
-                               f"{uc} K; errors in lockstep (one dt sequence)",
+                               f"{uc} K; errors "
+                               + ("in lockstep (one dt sequence)" if args.acc_mode == "lockstep"
+                                  else "at matched physical times, each form on its own dt sequence"),
                    "nx": nx, "ny": ny, "precision": args.acc_other},
         "fp64_value": round(v64, 3), "mixed_over_fp64": round(vmx / v64, 3), "timing": timing,
-        "lockstep_steps": args.acc_steps, "lockstep_broken_at": broken, "comparisons": rows,
+        "acc_mode": args.acc_mode, "lockstep_steps": args.acc_steps if args.acc_mode == "lockstep" else None,
+        "lockstep_broken_at": broken, "comparisons": rows,
         "max_dmu_J_per_mol": max(r["max_dmu_J_per_mol"] for r in rows),
         "max_dphi": [max(r["max_dphi"][i] for r in rows) for i in range(3)],
     }
@@ -551,6 +586,8 @@ def main():
                          "coefficients, residuals, dots, mass and acceptance FP64)")
     ap.add_argument("--acc-steps", type=int, default=3000, help="mixed-accuracy: FP64 steps in total")
     ap.add_argument("--acc-every", type=int, default=1000, help="mixed-accuracy: steps between comparisons")
+    ap.add_argument("--acc-mode", choices=["lockstep", "free"], default="lockstep",
+                    help="mixed-accuracy: one dt sequence, or each form on its own dt compared at matched times")
     ap.add_argument("--acc-other", choices=["mixed", "fp64-k8"], default="mixed",
                     help="mixed-accuracy: the form compared with FP64 at the default tile height; fp64-k8 is the "
                          "FP64 form with another summation order (8-row tiles), the noise floor of the comparison")
