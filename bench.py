@@ -724,6 +724,12 @@ def main():
             if v["launches"]:
                 log(f"  {name:16s} {v['launches']:6d} launches {v['ms']:9.3f} ms  "
                     f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s")
+    # Measured bandwidth ceiling beside the vendor peak: a hand-written plain
+    # copy of 1 GiB (pearl_stream_probe; read + write counted).
+    copy_gbs = None
+    if not args.no_profile:
+        copy_gbs = pkg.stream_probe(0, 1 << 30, 20)
+        log(f"copy ceiling: {copy_gbs:.1f} GB/s (1 GiB device copy, pearl_stream_probe)")
     mcs = nx * ny * args.steps / wall / 1e6
     rows_per_tile = s.layout()[1]
     roof = None
@@ -733,6 +739,8 @@ def main():
         achieved = dom[1]["bytes"] / (dom[1]["ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "copy_ceiling_GBps": round(copy_gbs, 1) if copy_gbs else None,
+                "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None,
                 "bytes_per_launch": dom[1]["bytes"] / dom[1]["launches"],
                 "avg_launch_ms": dom[1]["ms"] / dom[1]["launches"],
                 "region": f"the {args.kernel_steps} steps after the timed region, HIP events around each cell kernel",

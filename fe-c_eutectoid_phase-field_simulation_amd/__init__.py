@@ -166,7 +166,7 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
            "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance",
-           "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition"]
+           "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition", "pearl_stream_probe"]
 
 
 def lib():
@@ -191,6 +191,7 @@ def lib():
         L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
         L.pearl_fill_phases.argtypes = [vp, C.POINTER(Fill), C.c_int]
+        L.pearl_stream_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
         L.pearl_init_composition.argtypes = [vp, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
         L.pearl_step_fixed.argtypes = [vp, C.c_double]
         L.pearl_set_stop_tolerance.argtypes = [vp, C.c_double]
@@ -226,6 +227,16 @@ PARAM_KEYS = ["A0", "B0", "C0", "D0", "A1", "B1", "C1", "D1", "A2", "B2", "C2", 
 
 PAIR_SIGMA_KEYS = ("sigma_at", "sigma_ag", "sigma_tg")
 ADJOINT_GRAD = ("sigma", "tau_at", "tau_ag", "tau_tg", "D0", "D1", "D2", "sigma_at", "sigma_ag", "sigma_tg")
+
+
+def stream_probe(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """GB/s of a plain device copy of `nbytes` (pearl_stream_probe: read +
+    write counted), the measured bandwidth ceiling."""
+    g = C.c_double(0)
+    rc = lib().pearl_stream_probe(int(device), int(nbytes), int(reps), C.byref(g))
+    if rc != OK:
+        raise PearlError(rc, "pearl_stream_probe failed")
+    return g.value
 
 
 def params_from_dict(d: dict) -> Params:

@@ -1473,6 +1473,70 @@ const char* pearl_kernel_name(int kernel) {
     return kKernelNames[kernel];
 }
 
+}  // extern "C"
+
+namespace {
+typedef double probe_vec __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_probe(const probe_vec* __restrict__ a, probe_vec* __restrict__ b,
+                                                      long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+    else b[i] = a[i];
+}
+
+// Milliseconds of `reps` copies of n vectors with kernel variant NT.
+template <bool NT>
+hipError_t time_copies(const probe_vec* a, probe_vec* b, long long n, int reps, hipStream_t st, hipEvent_t e0,
+                       hipEvent_t e1, float* ms) {
+    const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+    for (int r = 0; r < 3; ++r) hipLaunchKernelGGL(k_stream_probe<NT>, grid, dim3(256), 0, st, a, b, n);
+    hipError_t e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(k_stream_probe<NT>, grid, dim3(256), 0, st, a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    return e;
+}
+}  // namespace
+
+extern "C" {
+
+int pearl_stream_probe(int device, long long bytes, int reps, double* gbps) {
+    if (!gbps || bytes <= 0 || bytes % 16 != 0 || bytes / 16 > (static_cast<long long>(1) << 38) || reps < 1)
+        return PEARL_ERR_ARGUMENT;
+    *gbps = 0;
+    if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
+    const long long n = bytes / 16;
+    probe_vec *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, static_cast<size_t>(bytes));
+    if (e == hipSuccess) e = hipMalloc(&b, static_cast<size_t>(bytes));
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, static_cast<size_t>(bytes), st);
+    float ms_plain = 0, ms_nt = 0;
+    if (e == hipSuccess) e = time_copies<false>(a, b, n, reps, st, e0, e1, &ms_plain);
+    if (e == hipSuccess) e = time_copies<true>(a, b, n, reps, st, e0, e1, &ms_nt);
+    if (e == hipSuccess) {
+        const double ms = std::min(ms_plain, ms_nt);
+        *gbps = ms > 0 ? 2.0 * static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9 : 0.0;
+    }
+    if (e1) (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (st) (void)hipStreamDestroy(st);
+    if (b) (void)hipFree(b);
+    if (a) (void)hipFree(a);
+    return e == hipSuccess ? PEARL_OK : (e == hipErrorOutOfMemory ? PEARL_ERR_ALLOC : PEARL_ERR_DEVICE);
+}
+
 int pearl_counters(const pearl_ctx* c, long long* attempts, long long* rejected, long long* cg) {
     if (!c) return PEARL_ERR_ARGUMENT;
     long long a = 0, r = 0, g = 0;
