@@ -69,3 +69,19 @@ def test_binding_structs_match_the_header(tmp_path):
     for i, name in enumerate(pkg.FILL_METHODS):
         assert defined["PEARL_FILL_" + name[4:]] == str(i), name
     assert defined["PEARL_FILL_PARAMS"] == str(len(pkg.Fill().p))
+
+
+def test_argument_checks_before_any_device_call():
+    """Entry points reject bad arguments before they touch a device, so these
+    calls run without a GPU."""
+    pkg = load_pkg()
+    L = pkg.lib()
+    g = ctypes.c_double(1.0)
+    for nbytes, reps in ((15, 1), (0, 1), (-16, 1), (1 << 20, 0)):
+        assert L.pearl_stream_probe(0, nbytes, reps, ctypes.byref(g)) == pkg.ERR_ARGUMENT, (nbytes, reps)
+    assert L.pearl_stream_probe(0, 1 << 20, 1, None) == pkg.ERR_ARGUMENT
+    # a null context is an argument error for every context call
+    assert L.pearl_fill_phases(None, None, 0) == pkg.ERR_ARGUMENT
+    mu, carbon, valid = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+    assert L.pearl_init_composition(None, 0, 1.0, ctypes.byref(mu), ctypes.byref(carbon),
+                                    ctypes.byref(valid)) == pkg.ERR_ARGUMENT
