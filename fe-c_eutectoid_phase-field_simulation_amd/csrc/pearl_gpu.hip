@@ -410,8 +410,9 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
         c->sc[d].rejected = h.rejected;
         peak = std::max(peak, h.cg_peak);
     }
-    // PCG slots per round: the largest solve plus the true-residual check.
-    if (peak > 0) c->cg_slots = std::min(24, std::max(3, peak + 2));
+    // PCG slots per round: the largest solve plus the true-residual check
+    // (a longer solve continues in the next round).
+    if (peak > 0) c->cg_slots = std::min(24, std::max(3, peak + 1));
     return PEARL_OK;
 }
 
@@ -999,7 +1000,7 @@ int run_fixed(pearl_ctx* c, const double* dts, int n, double* save = nullptr) {
         batch = std::max<long long>(1, std::min<long long>(64, h.target_steps - h.taken));
     }
     const int code = h.status == ST_OK ? PEARL_OK : status_of(c, 0);
-    if (h.cg_peak > 0) c->cg_slots = std::min(24, std::max(3, h.cg_peak + 2));
+    if (h.cg_peak > 0) c->cg_slots = std::min(24, std::max(3, h.cg_peak + 1));
     h.status = ST_OK;
     c->sc[0].time = h.time;
     c->sc[0].step = h.step;
