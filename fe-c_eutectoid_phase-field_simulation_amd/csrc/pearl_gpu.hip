@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <new>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -98,6 +99,17 @@ namespace {
 int fail(pearl_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     return code;
+}
+
+// Host allocation failed inside an entry point: no exception crosses the C ABI.
+int alloc_fail(pearl_ctx* c) noexcept {
+    if (c) {
+        try {
+            c->err = "host allocation failed";
+        } catch (...) {
+        }
+    }
+    return PEARL_ERR_ALLOC;
 }
 
 int hip_fail(pearl_ctx* c, hipError_t e, const char* where) {
@@ -534,96 +546,108 @@ extern "C" {
 
 int pearl_batch_create(int count, const pearl_params* params, const double* mu_references, int precision, int device,
                        int cells_per_thread, pearl_ctx** out) {
-    if (!out || count < 1 || !params || !mu_references) return PEARL_ERR_ARGUMENT;
-    *out = nullptr;
-    if (precision != PEARL_FP64 && precision != PEARL_MIXED) return PEARL_ERR_ARGUMENT;
-    for (int d = 0; d < count; ++d)
-        if (!params_ok(&params[d])) return PEARL_ERR_ARGUMENT;
-    // rows per tile; 4 measured best on the 4096x2048 workload (2, 8, 16
-    // within 2-7 % of it)
-    int k = cells_per_thread;
-    if (k <= 0) k = 4;
-    if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return PEARL_ERR_ARGUMENT;
-    if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
-    auto* c = new pearl_ctx();
-    c->device = device;
-    c->ndom = count;
-    c->k = k;
-    c->mx = precision == PEARL_MIXED ? 1 : 0;
-    c->prm.assign(params, params + count);
-    for (int d = 0; d < count; ++d) c->any_phase_only = c->any_phase_only || params[d].phase_only != 0;
-    c->sc.assign(count, pearl_scalars{});
-    c->runs_seen.assign(static_cast<size_t>(count) * NKIND, 0u);
-    // Pack the domains; each slice starts on a 64-cell (512-byte) boundary.
-    long long off = 0, boff = 0, poff = 0;
-    std::vector<int> blk2dom;
-    for (int d = 0; d < count; ++d) {
-        KP P = make_kp(&params[d], mu_references[d], k);
-        P.off = off;
-        P.boff = static_cast<int>(boff);
-        P.poff = poff;
-        off += (static_cast<long long>(P.n) + 63) / 64 * 64;
-        boff += P.nb;
-        poff += 8LL * P.nb + 8;
-        blk2dom.insert(blk2dom.end(), P.nb, d);
-        c->kp.push_back(P);
+    pearl_ctx* c = nullptr;
+    try {
+        if (!out || count < 1 || !params || !mu_references) return PEARL_ERR_ARGUMENT;
+        *out = nullptr;
+        if (precision != PEARL_FP64 && precision != PEARL_MIXED) return PEARL_ERR_ARGUMENT;
+        for (int d = 0; d < count; ++d)
+            if (!params_ok(&params[d])) return PEARL_ERR_ARGUMENT;
+        // rows per tile; 4 measured best on the 4096x2048 workload (2, 8, 16
+        // within 2-7 % of it)
+        int k = cells_per_thread;
+        if (k <= 0) k = 4;
+        if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return PEARL_ERR_ARGUMENT;
+        if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
+        c = new (std::nothrow) pearl_ctx();
+        if (!c) return PEARL_ERR_ALLOC;
+        c->device = device;
+        c->ndom = count;
+        c->k = k;
+        c->mx = precision == PEARL_MIXED ? 1 : 0;
+        c->prm.assign(params, params + count);
+        for (int d = 0; d < count; ++d) c->any_phase_only = c->any_phase_only || params[d].phase_only != 0;
+        c->sc.assign(count, pearl_scalars{});
+        c->runs_seen.assign(static_cast<size_t>(count) * NKIND, 0u);
+        auto bail0 = [&](int code) {
+            delete c;
+            return code;
+        };
+        // Pack the domains; each slice starts on a 64-cell (512-byte) boundary.
+        long long off = 0, boff = 0, poff = 0;
+        std::vector<int> blk2dom;
+        for (int d = 0; d < count; ++d) {
+            KP P = make_kp(&params[d], mu_references[d], k);
+            P.off = off;
+            P.boff = static_cast<int>(boff);
+            P.poff = poff;
+            off += (static_cast<long long>(P.n) + 63) / 64 * 64;
+            boff += P.nb;
+            if (boff > 2147483647LL) return bail0(PEARL_ERR_ARGUMENT);  // one launch grid holds every block
+            poff += 8LL * P.nb + 8;
+            blk2dom.insert(blk2dom.end(), P.nb, d);
+            c->kp.push_back(P);
+        }
+        c->total_cells = off;
+        c->total_blocks = boff;
+        c->total_parts = poff;
+        // Planes are spaced by the packed cells plus 576 cells (4.5 KiB): with a
+        // power-of-two domain the bare spacing would put the same cell of every
+        // plane on the same cache set and memory channel.
+        const size_t plane = static_cast<size_t>(off) + 576;
+        const size_t total = plane * (3 * 2 + 2 + 2 + 11) + static_cast<size_t>(poff) + 64;
+        auto bail = [&](int code) {
+            pearl_destroy(c);
+            return code;
+        };
+        if (hipMalloc(&c->mem, total * sizeof(double)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+        double* q = c->mem;
+        for (int b = 0; b < 2; ++b) {
+            c->B.phi[b] = q;
+            q += 3 * plane;
+        }
+        for (int b = 0; b < 2; ++b) {
+            c->B.c[b] = q;
+            q += plane;
+        }
+        for (int b = 0; b < 2; ++b) {
+            c->B.mu[b] = q;
+            q += plane;
+        }
+        double** work[] = {&c->B.chi, &c->B.icpt, &c->B.K, &c->B.rhs, &c->B.x, &c->B.diag,
+                           &c->B.r,   &c->B.z,    &c->B.p0, &c->B.p1, &c->B.Ap};
+        for (double** w : work) {
+            *w = q;
+            q += plane;
+        }
+        c->B.part = q;
+        if (hipMalloc(&c->d_kp, count * sizeof(KP)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+        if (hipMalloc(&c->d_S, count * sizeof(DevScal)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+        if (hipMalloc(&c->d_blk2dom, blk2dom.size() * sizeof(int)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+        if (hipMalloc(&c->d_kinds, MAX_SLOTS * sizeof(int)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+        if (hipHostMalloc(&c->hS, count * sizeof(DevScal)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(PEARL_ERR_DEVICE);
+        c->B.kp = c->d_kp;
+        c->B.S = c->d_S;
+        c->B.blk2dom = c->d_blk2dom;
+        c->B.kinds = c->d_kinds;
+        c->B.ndom = count;
+        std::memset(c->hS, 0, count * sizeof(DevScal));
+        for (int d = 0; d < count; ++d) c->hS[d].phase = PH_IDLE;
+        if (hipMemcpyAsync(c->d_kp, c->kp.data(), count * sizeof(KP), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->d_blk2dom, blk2dom.data(), blk2dom.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(c->d_S, c->hS, count * sizeof(DevScal), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->mem, 0, total * sizeof(double), c->stream) != hipSuccess ||
+            hipMemsetAsync(c->d_kinds, 0, MAX_SLOTS * sizeof(int), c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return bail(PEARL_ERR_DEVICE);
+        *out = c;
+        return PEARL_OK;
+    } catch (const std::bad_alloc&) {
+        if (c) pearl_destroy(c);  // the partly built context
+        return alloc_fail(nullptr);
     }
-    c->total_cells = off;
-    c->total_blocks = boff;
-    c->total_parts = poff;
-    // Planes are spaced by the packed cells plus 576 cells (4.5 KiB): with a
-    // power-of-two domain the bare spacing would put the same cell of every
-    // plane on the same cache set and memory channel.
-    const size_t plane = static_cast<size_t>(off) + 576;
-    const size_t total = plane * (3 * 2 + 2 + 2 + 11) + static_cast<size_t>(poff) + 64;
-    auto bail = [&](int code) {
-        pearl_destroy(c);
-        return code;
-    };
-    if (hipMalloc(&c->mem, total * sizeof(double)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
-    double* q = c->mem;
-    for (int b = 0; b < 2; ++b) {
-        c->B.phi[b] = q;
-        q += 3 * plane;
-    }
-    for (int b = 0; b < 2; ++b) {
-        c->B.c[b] = q;
-        q += plane;
-    }
-    for (int b = 0; b < 2; ++b) {
-        c->B.mu[b] = q;
-        q += plane;
-    }
-    double** work[] = {&c->B.chi, &c->B.icpt, &c->B.K, &c->B.rhs, &c->B.x, &c->B.diag,
-                       &c->B.r,   &c->B.z,    &c->B.p0, &c->B.p1, &c->B.Ap};
-    for (double** w : work) {
-        *w = q;
-        q += plane;
-    }
-    c->B.part = q;
-    if (hipMalloc(&c->d_kp, count * sizeof(KP)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
-    if (hipMalloc(&c->d_S, count * sizeof(DevScal)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
-    if (hipMalloc(&c->d_blk2dom, blk2dom.size() * sizeof(int)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
-    if (hipMalloc(&c->d_kinds, MAX_SLOTS * sizeof(int)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
-    if (hipHostMalloc(&c->hS, count * sizeof(DevScal)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(PEARL_ERR_DEVICE);
-    c->B.kp = c->d_kp;
-    c->B.S = c->d_S;
-    c->B.blk2dom = c->d_blk2dom;
-    c->B.kinds = c->d_kinds;
-    c->B.ndom = count;
-    std::memset(c->hS, 0, count * sizeof(DevScal));
-    for (int d = 0; d < count; ++d) c->hS[d].phase = PH_IDLE;
-    if (hipMemcpyAsync(c->d_kp, c->kp.data(), count * sizeof(KP), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->d_blk2dom, blk2dom.data(), blk2dom.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) !=
-            hipSuccess ||
-        hipMemcpyAsync(c->d_S, c->hS, count * sizeof(DevScal), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemsetAsync(c->mem, 0, total * sizeof(double), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->d_kinds, 0, MAX_SLOTS * sizeof(int), c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-        return bail(PEARL_ERR_DEVICE);
-    *out = c;
-    return PEARL_OK;
 }
 
 int pearl_create(const pearl_params* params, double mu_reference, int precision, int device, int cells_per_thread,
@@ -761,21 +785,29 @@ int single_run(pearl_ctx* c, double stop_time, long long max_steps, long long* t
 
 int pearl_batch_run(pearl_ctx* c, const double* stop_times, const long long* max_steps, long long* taken,
                     long long* cg_total, int* cg_peak, pearl_step_info* last, int* status) {
-    return batch_run(c, stop_times, max_steps, taken, cg_total, cg_peak, last, status, false);
+    try {
+        return batch_run(c, stop_times, max_steps, taken, cg_total, cg_peak, last, status, false);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
+    }
 }
 
 int pearl_run(pearl_ctx* c, double stop_time, long long max_steps, long long* taken, long long* cg_total, int* cg_peak,
               pearl_step_info* last) {
-    int status = 0;
-    pearl_step_info info{};
-    long long t = 0, cgt = cg_total ? *cg_total : 0;
-    int cgp = cg_peak ? *cg_peak : 0;
-    const int st = batch_run(c, &stop_time, &max_steps, &t, &cgt, &cgp, &info, &status, false);
-    if (taken) *taken = t;
-    if (cg_total) *cg_total = cgt;
-    if (cg_peak) *cg_peak = cgp;
-    if (last) *last = info;
-    return st;
+    try {
+        int status = 0;
+        pearl_step_info info{};
+        long long t = 0, cgt = cg_total ? *cg_total : 0;
+        int cgp = cg_peak ? *cg_peak : 0;
+        const int st = batch_run(c, &stop_time, &max_steps, &t, &cgt, &cgp, &info, &status, false);
+        if (taken) *taken = t;
+        if (cg_total) *cg_total = cgt;
+        if (cg_peak) *cg_peak = cgp;
+        if (last) *last = info;
+        return st;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
+    }
 }
 
 int pearl_set_stop_tolerance(pearl_ctx* c, double eps) {
@@ -786,170 +818,190 @@ int pearl_set_stop_tolerance(pearl_ctx* c, double eps) {
 }
 
 int pearl_advance(pearl_ctx* c, double stop_time, pearl_step_info* info) {
-    if (!c || !info) return PEARL_ERR_ARGUMENT;
-    long long taken = 0;
-    std::memset(info, 0, sizeof(*info));
-    return single_run(c, stop_time, 1, &taken, nullptr, nullptr, info, true);
+    try {
+        if (!c || !info) return PEARL_ERR_ARGUMENT;
+        long long taken = 0;
+        std::memset(info, 0, sizeof(*info));
+        return single_run(c, stop_time, 1, &taken, nullptr, nullptr, info, true);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
+    }
 }
 
 int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double target, double c_gamma, int* shifted,
                       double* front) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (!shifted) return PEARL_ERR_ARGUMENT;
-    *shifted = 0;
-    if (front) *front = -1;
-    if (!moving_window) return PEARL_OK;
-    const KP& P = c->kp[0];
-    PEARL_LAUNCH(c, k_front, dim3(P.nb), c->B, 0);
-    hipLaunchKernelGGL(k_front_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0);
-    st = read_scalars(c);
-    if (st) return st;
-    const double fr = c->hS[0].front;
-    if (front) *front = fr;
-    if (fr < trigger) return PEARL_OK;
-    const int rows = static_cast<int>(std::floor((fr - target) / P.dy));
-    const int scan_rows = std::max(0, std::min(rows, P.ny));
-    double removed = 0;
-    if (scan_rows > 0) {
-        const int nbs = P.tiles_x * ((scan_rows + c->k - 1) / c->k);
-        PEARL_LAUNCH(c, k_strip, dim3(nbs), c->B, 0, scan_rows, nbs);
-        hipLaunchKernelGGL(k_strip_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0, nbs);
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (!shifted) return PEARL_ERR_ARGUMENT;
+        *shifted = 0;
+        if (front) *front = -1;
+        if (!moving_window) return PEARL_OK;
+        const KP& P = c->kp[0];
+        PEARL_LAUNCH(c, k_front, dim3(P.nb), c->B, 0);
+        hipLaunchKernelGGL(k_front_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0);
         st = read_scalars(c);
         if (st) return st;
-        if (c->hS[0].gmax > 1e-6) return fail(c, PEARL_ERR_UNSAFE_SHIFT, "unsafe shift: removed strip contains gamma");
-        removed = c->hS[0].removed;
+        const double fr = c->hS[0].front;
+        if (front) *front = fr;
+        if (fr < trigger) return PEARL_OK;
+        const int rows = static_cast<int>(std::floor((fr - target) / P.dy));
+        const int scan_rows = std::max(0, std::min(rows, P.ny));
+        double removed = 0;
+        if (scan_rows > 0) {
+            const int nbs = P.tiles_x * ((scan_rows + c->k - 1) / c->k);
+            PEARL_LAUNCH(c, k_strip, dim3(nbs), c->B, 0, scan_rows, nbs);
+            hipLaunchKernelGGL(k_strip_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0, nbs);
+            st = read_scalars(c);
+            if (st) return st;
+            if (c->hS[0].gmax > 1e-6) return fail(c, PEARL_ERR_UNSAFE_SHIFT, "unsafe shift: removed strip contains gamma");
+            removed = c->hS[0].removed;
+        }
+        if (rows <= 0 || rows >= P.ny || c_gamma < 0 || c_gamma > 1) return fail(c, PEARL_ERR_INVALID_SHIFT, "invalid shift");
+        const double added = rows * P.nx * c_gamma * P.dx * P.dy;
+        PEARL_LAUNCH(c, k_shift, dim3(P.nb), c->B, 0, rows, c_gamma);
+        st = read_scalars(c);
+        if (st) return st;
+        DevScal& h = c->hS[0];
+        if (h.err & ERRF_MIXTURE) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+        if (h.err & ERRF_NONFINITE_CARBON) return fail(c, PEARL_ERR_NONFINITE_CARBON, "Nonfinite total carbon");
+        h.cur ^= 1;
+        h.e0_valid = 0;
+        h.m0_valid = 0;
+        st = write_scalars(c);
+        if (st) return st;
+        c->sc[0].shift_distance += rows * P.dy;
+        c->sc[0].shifts++;
+        c->sc[0].carbon_added += added;
+        c->sc[0].carbon_removed += removed;
+        *shifted = 1;
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return PEARL_OK;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    if (rows <= 0 || rows >= P.ny || c_gamma < 0 || c_gamma > 1) return fail(c, PEARL_ERR_INVALID_SHIFT, "invalid shift");
-    const double added = rows * P.nx * c_gamma * P.dx * P.dy;
-    PEARL_LAUNCH(c, k_shift, dim3(P.nb), c->B, 0, rows, c_gamma);
-    st = read_scalars(c);
-    if (st) return st;
-    DevScal& h = c->hS[0];
-    if (h.err & ERRF_MIXTURE) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
-    if (h.err & ERRF_NONFINITE_CARBON) return fail(c, PEARL_ERR_NONFINITE_CARBON, "Nonfinite total carbon");
-    h.cur ^= 1;
-    h.e0_valid = 0;
-    h.m0_valid = 0;
-    st = write_scalars(c);
-    if (st) return st;
-    c->sc[0].shift_distance += rows * P.dy;
-    c->sc[0].shifts++;
-    c->sc[0].carbon_added += added;
-    c->sc[0].carbon_removed += removed;
-    *shifted = 1;
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return PEARL_OK;
 }
 
 int pearl_diagnostics(pearl_ctx* c, pearl_diag* out) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (!out) return PEARL_ERR_ARGUMENT;
-    const KP& P = c->kp[0];
-    PEARL_LAUNCH(c, k_diagnostics, dim3(P.nb), c->B, 0);
-    hipLaunchKernelGGL(k_diagnostics_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0);
-    st = read_scalars(c);
-    if (st) return st;
-    const DevScal& h = c->hS[0];
-    if (!c->prm[0].phase_only && (h.err & ERRF_MIXTURE_STATE))
-        return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
-    out->c_min = h.dmin_c;
-    out->c_max = h.dmax_c;
-    out->mu_min = h.dmin_mu;
-    out->mu_max = h.dmax_mu;
-    out->phi_sum_error = h.dsumerr;
-    out->mass = h.dmass * P.dx * P.dy;
-    out->energy = h.denergy * P.dx * P.dy;
-    return PEARL_OK;
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (!out) return PEARL_ERR_ARGUMENT;
+        const KP& P = c->kp[0];
+        PEARL_LAUNCH(c, k_diagnostics, dim3(P.nb), c->B, 0);
+        hipLaunchKernelGGL(k_diagnostics_fin, dim3(1), dim3(TPB), 0, c->stream, c->B, 0);
+        st = read_scalars(c);
+        if (st) return st;
+        const DevScal& h = c->hS[0];
+        if (!c->prm[0].phase_only && (h.err & ERRF_MIXTURE_STATE))
+            return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+        out->c_min = h.dmin_c;
+        out->c_max = h.dmax_c;
+        out->mu_min = h.dmin_mu;
+        out->mu_max = h.dmax_mu;
+        out->phi_sum_error = h.dsumerr;
+        out->mass = h.dmass * P.dx * P.dy;
+        out->energy = h.denergy * P.dx * P.dy;
+        return PEARL_OK;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
+    }
 }
 
 int pearl_propose_phi(pearl_ctx* c, double dt, double* phi_out, double* max_change) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (!max_change) return PEARL_ERR_ARGUMENT;
-    st = read_scalars(c);
-    if (st) return st;
-    DevScal& h = c->hS[0];
-    h.phase = PH_PROPOSE;
-    h.need_state = 0;
-    h.dt = dt;
-    h.status = ST_OK;
-    st = write_scalars(c);
-    if (st) return st;
-    PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
-    hipLaunchKernelGGL(k_propose_fin, dim3(static_cast<unsigned>(c->ndom)), dim3(TPB), 0, c->stream, c->B);
-    st = read_scalars(c);
-    if (st) return st;
-    const int status = h.status;
-    *max_change = h.change;
-    h.phase = PH_IDLE;
-    h.status = ST_OK;
-    st = write_scalars(c);
-    if (st) return st;
-    if (status == ST_PROJECTION) return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
-    if (phi_out) {
-        HIP_CHECK(c, hipMemcpyAsync(phi_out, dom_phi(c, 0, h.cur ^ 1), 3 * static_cast<size_t>(c->kp[0].n) * sizeof(double),
-                                    hipMemcpyDeviceToHost, c->stream));
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (!max_change) return PEARL_ERR_ARGUMENT;
+        st = read_scalars(c);
+        if (st) return st;
+        DevScal& h = c->hS[0];
+        h.phase = PH_PROPOSE;
+        h.need_state = 0;
+        h.dt = dt;
+        h.status = ST_OK;
+        st = write_scalars(c);
+        if (st) return st;
+        PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
+        hipLaunchKernelGGL(k_propose_fin, dim3(static_cast<unsigned>(c->ndom)), dim3(TPB), 0, c->stream, c->B);
+        st = read_scalars(c);
+        if (st) return st;
+        const int status = h.status;
+        *max_change = h.change;
+        h.phase = PH_IDLE;
+        h.status = ST_OK;
+        st = write_scalars(c);
+        if (st) return st;
+        if (status == ST_PROJECTION) return fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+        if (phi_out) {
+            HIP_CHECK(c, hipMemcpyAsync(phi_out, dom_phi(c, 0, h.cur ^ 1), 3 * static_cast<size_t>(c->kp[0].n) * sizeof(double),
+                                        hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return PEARL_OK;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return PEARL_OK;
 }
 
 int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, double* carbon_out, double* mu_out,
                              pearl_diffusion_info* info) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (!phi_trial || !info || c->prm[0].phase_only) return PEARL_ERR_ARGUMENT;
-    st = read_scalars(c);
-    if (st) return st;
-    DevScal& h = c->hS[0];
-    const size_t n = static_cast<size_t>(c->kp[0].n);
-    HIP_CHECK(c, hipMemcpyAsync(dom_phi(c, 0, h.cur ^ 1), phi_trial, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    h.phase = PH_SOLVE;
-    h.live = 1;
-    h.cg_state = CG_IDLE;
-    h.cg_it = 0;
-    h.cg_res = 0;
-    h.closure = 0;
-    h.pad = 0;
-    h.dt = dt;
-    h.status = ST_OK;
-    h.err = 0;
-    st = write_scalars(c);
-    if (st) return st;
-    // rounds without begin/decide until the solve leaves PH_SOLVE
-    for (int guard = 0; guard < 100000; ++guard) {
-        if (c->prof) {
-            hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
-            enqueue_round(c, c->cg_slots, true, false, false, false);
-        } else {
-            enqueue_round(c, c->cg_slots, false, false, false, false);
-        }
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (!phi_trial || !info || c->prm[0].phase_only) return PEARL_ERR_ARGUMENT;
         st = read_scalars(c);
         if (st) return st;
-        if (h.phase != PH_SOLVE) break;
-    }
-    if (h.status != ST_OK) {
-        const int code = status_of(c, 0);
-        h.phase = PH_IDLE;
+        DevScal& h = c->hS[0];
+        const size_t n = static_cast<size_t>(c->kp[0].n);
+        HIP_CHECK(c, hipMemcpyAsync(dom_phi(c, 0, h.cur ^ 1), phi_trial, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        h.phase = PH_SOLVE;
+        h.live = 1;
+        h.cg_state = CG_IDLE;
+        h.cg_it = 0;
+        h.cg_res = 0;
+        h.closure = 0;
+        h.pad = 0;
+        h.dt = dt;
         h.status = ST_OK;
-        (void)write_scalars(c);
-        return code;
+        h.err = 0;
+        st = write_scalars(c);
+        if (st) return st;
+        // rounds without begin/decide until the solve leaves PH_SOLVE
+        for (int guard = 0; guard < 100000; ++guard) {
+            if (c->prof) {
+                hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
+                enqueue_round(c, c->cg_slots, true, false, false, false);
+            } else {
+                enqueue_round(c, c->cg_slots, false, false, false, false);
+            }
+            st = read_scalars(c);
+            if (st) return st;
+            if (h.phase != PH_SOLVE) break;
+        }
+        if (h.status != ST_OK) {
+            const int code = status_of(c, 0);
+            h.phase = PH_IDLE;
+            h.status = ST_OK;
+            (void)write_scalars(c);
+            return code;
+        }
+        const bool finalized = h.cg_state == CG_OK;
+        info->it = h.cg_it;
+        info->residual = h.cg_res;
+        info->closure = finalized ? h.closure : 0;
+        info->ok = finalized && info->closure <= c->prm[0].closure_tol;
+        h.phase = PH_IDLE;
+        st = write_scalars(c);
+        if (st) return st;
+        const int src = finalized ? (h.cur ^ 1) : h.cur;
+        if (carbon_out) HIP_CHECK(c, hipMemcpyAsync(carbon_out, dom_c(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (mu_out) HIP_CHECK(c, hipMemcpyAsync(mu_out, dom_mu(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return PEARL_OK;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    const bool finalized = h.cg_state == CG_OK;
-    info->it = h.cg_it;
-    info->residual = h.cg_res;
-    info->closure = finalized ? h.closure : 0;
-    info->ok = finalized && info->closure <= c->prm[0].closure_tol;
-    h.phase = PH_IDLE;
-    st = write_scalars(c);
-    if (st) return st;
-    const int src = finalized ? (h.cur ^ 1) : h.cur;
-    if (carbon_out) HIP_CHECK(c, hipMemcpyAsync(carbon_out, dom_c(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (mu_out) HIP_CHECK(c, hipMemcpyAsync(mu_out, dom_mu(c, 0, src), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return PEARL_OK;
 }
 
 namespace {
@@ -1016,28 +1068,36 @@ int step_fixed(pearl_ctx* c, double dt) {
 }  // namespace
 
 int pearl_step_fixed(pearl_ctx* c, double dt) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (c->prm[0].phase_only || c->mx || !(dt > 0) || !std::isfinite(dt)) return PEARL_ERR_ARGUMENT;
-    return step_fixed(c, dt);
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (c->prm[0].phase_only || c->mx || !(dt > 0) || !std::isfinite(dt)) return PEARL_ERR_ARGUMENT;
+        return step_fixed(c, dt);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
+    }
 }
 
 int pearl_run_fixed(pearl_ctx* c, int steps, const double* dts) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (c->ndom != 1 || c->prm[0].phase_only || c->mx || steps < 1 || !dts) return PEARL_ERR_ARGUMENT;
-    for (int k = 0; k < steps; ++k)
-        if (!(dts[k] > 0) || !std::isfinite(dts[k])) return PEARL_ERR_ARGUMENT;
-    if (steps > c->d_dtn_cap) {
-        if (c->d_dtn) (void)hipFree(c->d_dtn);
-        c->d_dtn = nullptr;
-        c->d_dtn_cap = 0;
-        if (hipMalloc(&c->d_dtn, steps * sizeof(double)) != hipSuccess)
-            return fail(c, PEARL_ERR_ALLOC, "out of device memory");
-        c->d_dtn_cap = steps;
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (c->ndom != 1 || c->prm[0].phase_only || c->mx || steps < 1 || !dts) return PEARL_ERR_ARGUMENT;
+        for (int k = 0; k < steps; ++k)
+            if (!(dts[k] > 0) || !std::isfinite(dts[k])) return PEARL_ERR_ARGUMENT;
+        if (steps > c->d_dtn_cap) {
+            if (c->d_dtn) (void)hipFree(c->d_dtn);
+            c->d_dtn = nullptr;
+            c->d_dtn_cap = 0;
+            if (hipMalloc(&c->d_dtn, steps * sizeof(double)) != hipSuccess)
+                return fail(c, PEARL_ERR_ALLOC, "out of device memory");
+            c->d_dtn_cap = steps;
+        }
+        HIP_CHECK(c, hipMemcpyAsync(c->d_dtn, dts, steps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        return run_fixed(c, c->d_dtn, steps);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    HIP_CHECK(c, hipMemcpyAsync(c->d_dtn, dts, steps * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    return run_fixed(c, c->d_dtn, steps);
 }
 
 namespace {
@@ -1090,233 +1150,241 @@ int adjoint_solve(pearl_ctx* c, const Adj& A) {
 
 int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const double* phi_target, double w_front,
                   double front_target, double* loss, double* grad) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (c->ndom != 1 || c->mx || c->prm[0].phase_only || steps < 1 || every < 1 || !dts || !phi_target || !loss ||
-        !grad || !std::isfinite(w_front) || w_front < 0)
-        return PEARL_ERR_ARGUMENT;
-    for (int k = 0; k < steps; ++k)
-        if (!(dts[k] > 0) || !std::isfinite(dts[k])) return PEARL_ERR_ARGUMENT;
-    const KP& P = c->kp[0];
-    const size_t n = static_cast<size_t>(P.n);
-    const int segs = (steps + every - 1) / every;
-    const int seg_len = std::min(every, steps);
-    // device memory: checkpoints, the states of one segment, the final state,
-    // two gradient sets, the trial-phase gradient, v and the accumulators
-    const size_t state = 5 * n;
-    const size_t total =
-        state * (static_cast<size_t>(segs) + seg_len + 1 + 1) + 2 * state + 6 * n + ADJ_THETA + static_cast<size_t>(steps);
-    double* mem = nullptr;
-    if (hipMalloc(&mem, total * sizeof(double)) != hipSuccess) return fail(c, PEARL_ERR_ALLOC, "adjoint: out of memory");
-    struct Free {
-        double* p;
-        ~Free() { (void)hipFree(p); }
-    } guard{mem};
-    double* ck = mem;
-    double* seg = ck + state * segs;
-    double* fin = seg + state * (seg_len + 1);
-    double* G[2] = {fin + state, fin + 2 * state};
-    double* gphit = G[1] + state;
-    double* vbuf = gphit + 3 * n;
-    double* theta = vbuf + 3 * n;
-    double* d_dts = theta + ADJ_THETA;
-    auto ph = [&](double* s5) { return s5; };
-    auto cc = [&](double* s5) { return s5 + 3 * n; };
-    auto mm = [&](double* s5) { return s5 + 4 * n; };
-    st = read_scalars(c);
-    if (st) return st;
-    DevScal& h = c->hS[0];
-    auto save = [&](double* s5) {
-        return copy_state(c, ph(s5), cc(s5), mm(s5), dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur));
-    };
-    auto load = [&](double* s5) {
-        return copy_state(c, dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur), ph(s5), cc(s5), mm(s5));
-    };
-    HIP_CHECK(c, hipMemcpyAsync(d_dts, dts, steps * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    // forward with checkpoints: one device-controlled run per segment
-    if ((st = save(ck))) return st;
-    for (int sgi = 0; sgi < segs; ++sgi) {
-        const int s0 = sgi * every, len = std::min(every, steps - s0);
-        if ((st = run_fixed(c, d_dts + s0, len))) return st;
-        if (s0 + len < steps && (st = save(ck + state * (sgi + 1)))) return st;
-    }
-    if ((st = save(fin))) return st;
-    const double t_end = h.time;
-    const long long step_end = h.step;
-    // loss and its gradient w.r.t. the final state (host)
-    std::vector<double> phiN(3 * n), g(5 * n, 0.0);
-    HIP_CHECK(c, hipMemcpyAsync(phiN.data(), ph(fin), 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    // compensated (Neumaier) sums: the loss and the front height do not
-    // depend on the summation order beyond the last bits
-    struct Sum {
-        double s = 0, c = 0;
-        void add(double v) {
-            const double t = s + v;
-            c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
-            s = t;
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (c->ndom != 1 || c->mx || c->prm[0].phase_only || steps < 1 || every < 1 || !dts || !phi_target || !loss ||
+            !grad || !std::isfinite(w_front) || w_front < 0)
+            return PEARL_ERR_ARGUMENT;
+        for (int k = 0; k < steps; ++k)
+            if (!(dts[k] > 0) || !std::isfinite(dts[k])) return PEARL_ERR_ARGUMENT;
+        const KP& P = c->kp[0];
+        const size_t n = static_cast<size_t>(P.n);
+        const int segs = (steps + every - 1) / every;
+        const int seg_len = std::min(every, steps);
+        // device memory: checkpoints, the states of one segment, the final state,
+        // two gradient sets, the trial-phase gradient, v and the accumulators
+        const size_t state = 5 * n;
+        const size_t total =
+            state * (static_cast<size_t>(segs) + seg_len + 1 + 1) + 2 * state + 6 * n + ADJ_THETA + static_cast<size_t>(steps);
+        double* mem = nullptr;
+        if (hipMalloc(&mem, total * sizeof(double)) != hipSuccess) return fail(c, PEARL_ERR_ALLOC, "adjoint: out of memory");
+        struct Free {
+            double* p;
+            ~Free() { (void)hipFree(p); }
+        } guard{mem};
+        double* ck = mem;
+        double* seg = ck + state * segs;
+        double* fin = seg + state * (seg_len + 1);
+        double* G[2] = {fin + state, fin + 2 * state};
+        double* gphit = G[1] + state;
+        double* vbuf = gphit + 3 * n;
+        double* theta = vbuf + 3 * n;
+        double* d_dts = theta + ADJ_THETA;
+        auto ph = [&](double* s5) { return s5; };
+        auto cc = [&](double* s5) { return s5 + 3 * n; };
+        auto mm = [&](double* s5) { return s5 + 4 * n; };
+        st = read_scalars(c);
+        if (st) return st;
+        DevScal& h = c->hS[0];
+        auto save = [&](double* s5) {
+            return copy_state(c, ph(s5), cc(s5), mm(s5), dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur));
+        };
+        auto load = [&](double* s5) {
+            return copy_state(c, dom_phi(c, 0, h.cur), dom_c(c, 0, h.cur), dom_mu(c, 0, h.cur), ph(s5), cc(s5), mm(s5));
+        };
+        HIP_CHECK(c, hipMemcpyAsync(d_dts, dts, steps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        // forward with checkpoints: one device-controlled run per segment
+        if ((st = save(ck))) return st;
+        for (int sgi = 0; sgi < segs; ++sgi) {
+            const int s0 = sgi * every, len = std::min(every, steps - s0);
+            if ((st = run_fixed(c, d_dts + s0, len))) return st;
+            if (s0 + len < steps && (st = save(ck + state * (sgi + 1)))) return st;
         }
-        double value() const { return s + c; }
-    } l2s, hs;
-    for (size_t i = 0; i < 3 * n; ++i) {
-        const double d = phiN[i] - phi_target[i];
-        l2s.add(d * d);
-        g[i] = d;
-    }
-    for (size_t i = 0; i < n; ++i) hs.add(1.0 - phiN[2 * n + i]);
-    const double l2 = l2s.value();
-    const double height = hs.value() * P.dy / P.nx;
-    const double dh_ = height - front_target;
-    *loss = 0.5 * l2 + 0.5 * w_front * dh_ * dh_;
-    for (size_t i = 0; i < n; ++i) g[2 * n + i] -= w_front * dh_ * P.dy / P.nx;
-    HIP_CHECK(c, hipMemcpyAsync(G[1], g.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(c, hipMemsetAsync(theta, 0, ADJ_THETA * sizeof(double), c->stream));
-    // reverse sweep, one checkpointed segment at a time
-    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
-    int cur = 1;  // G[cur] holds the gradient w.r.t. the later state
-    for (int sgi = segs - 1; sgi >= 0; --sgi) {
-        const int s0 = sgi * every, len = std::min(every, steps - s0);
-        if ((st = load(ck + state * sgi))) return st;
-        if ((st = save(seg))) return st;
-        if ((st = run_fixed(c, d_dts + s0, len, seg + state))) return st;
-        for (int k = len - 1; k >= 0; --k) {
-            double* s_n = seg + state * k;
-            double* s_n1 = seg + state * (k + 1);
-            double* gin = G[cur];
-            double* gout = G[cur ^ 1];
-            Adj A{ph(s_n), mm(s_n), ph(s_n1), mm(s_n1), ph(gin), cc(gin), mm(gin), ph(gout), cc(gout), mm(gout),
-                  gphit, vbuf, theta, dts[s0 + k]};
-            if ((st = adjoint_solve(c, A))) return st;
-            PEARL_LAUNCH(c, k_adj_diffgrad, grid, c->B, A);
-            hipLaunchKernelGGL(k_adj_diffgrad_fin, one, tb, 0, c->stream, c->B, A);
-            PEARL_LAUNCH(c, k_adj_propose_a, grid, c->B, A);
-            hipLaunchKernelGGL(k_adj_propose_a_fin, one, tb, 0, c->stream, c->B, A);
-            PEARL_LAUNCH(c, k_adj_propose_b, grid, c->B, A);
-            cur ^= 1;
+        if ((st = save(fin))) return st;
+        const double t_end = h.time;
+        const long long step_end = h.step;
+        // loss and its gradient w.r.t. the final state (host)
+        std::vector<double> phiN(3 * n), g(5 * n, 0.0);
+        HIP_CHECK(c, hipMemcpyAsync(phiN.data(), ph(fin), 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        // compensated (Neumaier) sums: the loss and the front height do not
+        // depend on the summation order beyond the last bits
+        struct Sum {
+            double s = 0, c = 0;
+            void add(double v) {
+                const double t = s + v;
+                c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+                s = t;
+            }
+            double value() const { return s + c; }
+        } l2s, hs;
+        for (size_t i = 0; i < 3 * n; ++i) {
+            const double d = phiN[i] - phi_target[i];
+            l2s.add(d * d);
+            g[i] = d;
         }
+        for (size_t i = 0; i < n; ++i) hs.add(1.0 - phiN[2 * n + i]);
+        const double l2 = l2s.value();
+        const double height = hs.value() * P.dy / P.nx;
+        const double dh_ = height - front_target;
+        *loss = 0.5 * l2 + 0.5 * w_front * dh_ * dh_;
+        for (size_t i = 0; i < n; ++i) g[2 * n + i] -= w_front * dh_ * P.dy / P.nx;
+        HIP_CHECK(c, hipMemcpyAsync(G[1], g.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(c, hipMemsetAsync(theta, 0, ADJ_THETA * sizeof(double), c->stream));
+        // reverse sweep, one checkpointed segment at a time
+        const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+        int cur = 1;  // G[cur] holds the gradient w.r.t. the later state
+        for (int sgi = segs - 1; sgi >= 0; --sgi) {
+            const int s0 = sgi * every, len = std::min(every, steps - s0);
+            if ((st = load(ck + state * sgi))) return st;
+            if ((st = save(seg))) return st;
+            if ((st = run_fixed(c, d_dts + s0, len, seg + state))) return st;
+            for (int k = len - 1; k >= 0; --k) {
+                double* s_n = seg + state * k;
+                double* s_n1 = seg + state * (k + 1);
+                double* gin = G[cur];
+                double* gout = G[cur ^ 1];
+                Adj A{ph(s_n), mm(s_n), ph(s_n1), mm(s_n1), ph(gin), cc(gin), mm(gin), ph(gout), cc(gout), mm(gout),
+                      gphit, vbuf, theta, dts[s0 + k]};
+                if ((st = adjoint_solve(c, A))) return st;
+                PEARL_LAUNCH(c, k_adj_diffgrad, grid, c->B, A);
+                hipLaunchKernelGGL(k_adj_diffgrad_fin, one, tb, 0, c->stream, c->B, A);
+                PEARL_LAUNCH(c, k_adj_propose_a, grid, c->B, A);
+                hipLaunchKernelGGL(k_adj_propose_a_fin, one, tb, 0, c->stream, c->B, A);
+                PEARL_LAUNCH(c, k_adj_propose_b, grid, c->B, A);
+                cur ^= 1;
+            }
+        }
+        double th[ADJ_THETA];
+        HIP_CHECK(c, hipMemcpyAsync(th, theta, ADJ_THETA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        // the context ends on the forward's final state
+        if ((st = load(fin))) return st;
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        HIP_CHECK(c, hipGetLastError());
+        // grad: sigma (the three pair energies moved together), tau_at, tau_ag,
+        // tau_tg, D_alpha, D_theta, D_gamma, then sigma_at, sigma_ag, sigma_tg
+        grad[0] = th[0] + th[1] + th[2];
+        for (int q = 0; q < 6; ++q) grad[1 + q] = th[3 + q];
+        for (int q = 0; q < 3; ++q) grad[7 + q] = th[q];
+        st = read_scalars(c);
+        if (st) return st;
+        h.time = t_end;
+        h.step = step_end;
+        h.phase = PH_IDLE;
+        h.e0_valid = 0;
+        h.m0_valid = 0;
+        st = write_scalars(c);
+        if (st) return st;
+        c->sc[0].time = t_end;
+        c->sc[0].step = step_end;
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return PEARL_OK;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    double th[ADJ_THETA];
-    HIP_CHECK(c, hipMemcpyAsync(th, theta, ADJ_THETA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    // the context ends on the forward's final state
-    if ((st = load(fin))) return st;
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    HIP_CHECK(c, hipGetLastError());
-    // grad: sigma (the three pair energies moved together), tau_at, tau_ag,
-    // tau_tg, D_alpha, D_theta, D_gamma, then sigma_at, sigma_ag, sigma_tg
-    grad[0] = th[0] + th[1] + th[2];
-    for (int q = 0; q < 6; ++q) grad[1 + q] = th[3 + q];
-    for (int q = 0; q < 3; ++q) grad[7 + q] = th[q];
-    st = read_scalars(c);
-    if (st) return st;
-    h.time = t_end;
-    h.step = step_end;
-    h.phase = PH_IDLE;
-    h.e0_valid = 0;
-    h.m0_valid = 0;
-    st = write_scalars(c);
-    if (st) return st;
-    c->sc[0].time = t_end;
-    c->sc[0].step = step_end;
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return PEARL_OK;
 }
 
 int pearl_smooth(pearl_ctx* c, int max_steps, int min_steps, double energy_rtol, double max_phase_change,
                  pearl_smooth_report* rep) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (!rep) return PEARL_ERR_ARGUMENT;
-    std::memset(rep, 0, sizeof(*rep));
-    if (max_steps < 1 || min_steps < 0 || min_steps > max_steps || !(energy_rtol > 0) || !(max_phase_change > 0) ||
-        max_phase_change > .2)
-        return fail(c, PEARL_ERR_ARGUMENT, "invalid sharp-smoothing controls");
-    st = read_scalars(c);
-    if (st) return st;
-    const KP& P = c->kp[0];
-    const pearl_params& prm = c->prm[0];
-    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
-    DevScal& h = c->hS[0];
-    const int cur0 = h.cur;
-    // interface energy of the current phases (:880)
-    PEARL_LAUNCH(c, k_iface_energy, grid, c->B, 0);
-    hipLaunchKernelGGL(k_iface_energy_fin, one, tb, 0, c->stream, c->B, 0);
-    st = read_scalars(c);
-    if (st) return st;
-    rep->energy_initial = h.sm_energy * P.dx * P.dy;
-    double previous = rep->energy_initial;
-    const double eps = prm.eps, sigma = prm.sigma;
-    const double base = .12 * std::min(prm.dx * prm.dx, prm.dy * prm.dy) / (eps * sigma);  // :884
-    int result = PEARL_OK;
-    for (int step = 1; step <= max_steps; ++step) {
-        PEARL_LAUNCH(c, k_smooth_force, grid, c->B, 0);
-        hipLaunchKernelGGL(k_smooth_force_fin, one, tb, 0, c->stream, c->B, 0);
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (!rep) return PEARL_ERR_ARGUMENT;
+        std::memset(rep, 0, sizeof(*rep));
+        if (max_steps < 1 || min_steps < 0 || min_steps > max_steps || !(energy_rtol > 0) || !(max_phase_change > 0) ||
+            max_phase_change > .2)
+            return fail(c, PEARL_ERR_ARGUMENT, "invalid sharp-smoothing controls");
         st = read_scalars(c);
         if (st) return st;
-        const double fmax = h.sm_fmax;
-        if (!std::isfinite(fmax)) {
-            result = fail(c, PEARL_ERR_SMOOTHING, "nonfinite smoothing force");
-            break;
-        }
-        if (fmax == 0) {
-            rep->steps = step - 1;
-            rep->converged = 1;
-            rep->final_relative_energy_change = 0;
-            rep->final_max_phase_change = 0;
-            break;
-        }
-        double scale = std::min(base, max_phase_change / fmax);
-        bool accepted = false;
-        double e_new = previous, change = 0;
-        for (int bt = 0; bt < 24 && !accepted; ++bt) {
-            PEARL_LAUNCH(c, k_smooth_trial, grid, c->B, 0, scale);
-            hipLaunchKernelGGL(k_smooth_trial_fin, one, tb, 0, c->stream, c->B, 0);
+        const KP& P = c->kp[0];
+        const pearl_params& prm = c->prm[0];
+        const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+        DevScal& h = c->hS[0];
+        const int cur0 = h.cur;
+        // interface energy of the current phases (:880)
+        PEARL_LAUNCH(c, k_iface_energy, grid, c->B, 0);
+        hipLaunchKernelGGL(k_iface_energy_fin, one, tb, 0, c->stream, c->B, 0);
+        st = read_scalars(c);
+        if (st) return st;
+        rep->energy_initial = h.sm_energy * P.dx * P.dy;
+        double previous = rep->energy_initial;
+        const double eps = prm.eps, sigma = prm.sigma;
+        const double base = .12 * std::min(prm.dx * prm.dx, prm.dy * prm.dy) / (eps * sigma);  // :884
+        int result = PEARL_OK;
+        for (int step = 1; step <= max_steps; ++step) {
+            PEARL_LAUNCH(c, k_smooth_force, grid, c->B, 0);
+            hipLaunchKernelGGL(k_smooth_force_fin, one, tb, 0, c->stream, c->B, 0);
             st = read_scalars(c);
             if (st) return st;
-            if (h.sm_bad > 0) {
-                result = fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+            const double fmax = h.sm_fmax;
+            if (!std::isfinite(fmax)) {
+                result = fail(c, PEARL_ERR_SMOOTHING, "nonfinite smoothing force");
                 break;
             }
-            change = h.sm_change;
-            e_new = h.sm_energy * P.dx * P.dy;
-            const double etol = 1e-14 * std::max({1.0, std::abs(previous), std::abs(rep->energy_initial)});
-            if (std::isfinite(e_new) && e_new <= previous + etol) accepted = true;
-            else scale *= .5;
+            if (fmax == 0) {
+                rep->steps = step - 1;
+                rep->converged = 1;
+                rep->final_relative_energy_change = 0;
+                rep->final_max_phase_change = 0;
+                break;
+            }
+            double scale = std::min(base, max_phase_change / fmax);
+            bool accepted = false;
+            double e_new = previous, change = 0;
+            for (int bt = 0; bt < 24 && !accepted; ++bt) {
+                PEARL_LAUNCH(c, k_smooth_trial, grid, c->B, 0, scale);
+                hipLaunchKernelGGL(k_smooth_trial_fin, one, tb, 0, c->stream, c->B, 0);
+                st = read_scalars(c);
+                if (st) return st;
+                if (h.sm_bad > 0) {
+                    result = fail(c, PEARL_ERR_PROJECTION, "Simplex projection failed");
+                    break;
+                }
+                change = h.sm_change;
+                e_new = h.sm_energy * P.dx * P.dy;
+                const double etol = 1e-14 * std::max({1.0, std::abs(previous), std::abs(rep->energy_initial)});
+                if (std::isfinite(e_new) && e_new <= previous + etol) accepted = true;
+                else scale *= .5;
+            }
+            if (result) break;
+            if (!accepted) {
+                result = fail(c, PEARL_ERR_SMOOTHING, "sharp smoothing could not find an energy-decreasing step");
+                break;
+            }
+            // the trial phases become current
+            h.cur ^= 1;
+            st = write_scalars(c);
+            if (st) return st;
+            const double rel = std::abs(previous - e_new) / std::max(std::abs(previous), 1e-300);
+            rep->steps = step;
+            rep->energy_final = e_new;
+            rep->final_relative_energy_change = rel;
+            rep->final_max_phase_change = change;
+            previous = e_new;
+            if (step >= min_steps && rel < energy_rtol && change < 1e-5) {
+                rep->converged = 1;
+                break;
+            }
         }
-        if (result) break;
-        if (!accepted) {
-            result = fail(c, PEARL_ERR_SMOOTHING, "sharp smoothing could not find an energy-decreasing step");
-            break;
+        if (rep->steps == 0) rep->energy_final = rep->energy_initial;
+        // carbon and mu stay where they were: follow the phases to the other buffer
+        if (h.cur != cur0) {
+            const size_t n = static_cast<size_t>(P.n);
+            HIP_CHECK(c, hipMemcpyAsync(dom_c(c, 0, h.cur), dom_c(c, 0, cur0), n * sizeof(double), hipMemcpyDeviceToDevice,
+                                        c->stream));
+            HIP_CHECK(c, hipMemcpyAsync(dom_mu(c, 0, h.cur), dom_mu(c, 0, cur0), n * sizeof(double),
+                                        hipMemcpyDeviceToDevice, c->stream));
         }
-        // the trial phases become current
-        h.cur ^= 1;
+        h.e0_valid = 0;
+        h.m0_valid = 0;
         st = write_scalars(c);
         if (st) return st;
-        const double rel = std::abs(previous - e_new) / std::max(std::abs(previous), 1e-300);
-        rep->steps = step;
-        rep->energy_final = e_new;
-        rep->final_relative_energy_change = rel;
-        rep->final_max_phase_change = change;
-        previous = e_new;
-        if (step >= min_steps && rel < energy_rtol && change < 1e-5) {
-            rep->converged = 1;
-            break;
-        }
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        return result;
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    if (rep->steps == 0) rep->energy_final = rep->energy_initial;
-    // carbon and mu stay where they were: follow the phases to the other buffer
-    if (h.cur != cur0) {
-        const size_t n = static_cast<size_t>(P.n);
-        HIP_CHECK(c, hipMemcpyAsync(dom_c(c, 0, h.cur), dom_c(c, 0, cur0), n * sizeof(double), hipMemcpyDeviceToDevice,
-                                    c->stream));
-        HIP_CHECK(c, hipMemcpyAsync(dom_mu(c, 0, h.cur), dom_mu(c, 0, cur0), n * sizeof(double),
-                                    hipMemcpyDeviceToDevice, c->stream));
-    }
-    h.e0_valid = 0;
-    h.m0_valid = 0;
-    st = write_scalars(c);
-    if (st) return st;
-    HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    return result;
 }
 
 static_assert(sizeof(FillOp) == sizeof(pearl_fill) && offsetof(FillOp, p) == offsetof(pearl_fill, p) &&
@@ -1329,88 +1397,96 @@ static_assert(FILL_CUBE == PEARL_FILL_CUBE && FILL_CYLINDER == PEARL_FILL_CYLIND
               "method codes of the C ABI");
 
 int pearl_fill_phases(pearl_ctx* c, const pearl_fill* ops, int count) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if (count < 0 || (count > 0 && !ops)) return fail(c, PEARL_ERR_ARGUMENT, "invalid FILL operation list");
-    // What the kernel relies on: a known method, phase IDs 0..2 where a
-    // label is taken from a parameter, finite parameters and directions, and
-    // at least one pair on a pearlite arm.  The rest is read_config's.
-    for (int k = 0; k < count; ++k) {
-        const pearl_fill& f = ops[k];
-        bool ok = f.method >= 0 && f.method < FILL_METHODS;
-        for (double v : f.p) ok = ok && std::isfinite(v);
-        for (const auto& u : f.dir) ok = ok && std::isfinite(u[0]) && std::isfinite(u[1]);
-        auto phase = [&](int i) { ok = ok && (f.p[i] == 0.0 || f.p[i] == 1.0 || f.p[i] == 2.0); };
-        if (ok) {
-            switch (f.method) {
-                case FILL_CUBE:
-                case FILL_CYLINDER: phase(0); break;
-                case FILL_CENTERBOX: phase(4), phase(5); break;
-                case FILL_YJUNCTION: phase(11), phase(12); break;
-                case FILL_YJUNCTION_LAMELLAE: phase(13), phase(14), phase(15); break;
-                default: phase(9), phase(10); break;  // pearlite arm, ellipse
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if (count < 0 || (count > 0 && !ops)) return fail(c, PEARL_ERR_ARGUMENT, "invalid FILL operation list");
+        // What the kernel relies on: a known method, phase IDs 0..2 where a
+        // label is taken from a parameter, finite parameters and directions, and
+        // at least one pair on a pearlite arm.  The rest is read_config's.
+        for (int k = 0; k < count; ++k) {
+            const pearl_fill& f = ops[k];
+            bool ok = f.method >= 0 && f.method < FILL_METHODS;
+            for (double v : f.p) ok = ok && std::isfinite(v);
+            for (const auto& u : f.dir) ok = ok && std::isfinite(u[0]) && std::isfinite(u[1]);
+            auto phase = [&](int i) { ok = ok && (f.p[i] == 0.0 || f.p[i] == 1.0 || f.p[i] == 2.0); };
+            if (ok) {
+                switch (f.method) {
+                    case FILL_CUBE:
+                    case FILL_CYLINDER: phase(0); break;
+                    case FILL_CENTERBOX: phase(4), phase(5); break;
+                    case FILL_YJUNCTION: phase(11), phase(12); break;
+                    case FILL_YJUNCTION_LAMELLAE: phase(13), phase(14), phase(15); break;
+                    default: phase(9), phase(10); break;  // pearlite arm, ellipse
+                }
+                if (f.method == FILL_ADHERED_ARM) ok = ok && f.p[4] >= 1.0 && f.p[4] <= 2147483647.0;
             }
-            if (f.method == FILL_ADHERED_ARM) ok = ok && f.p[4] >= 1.0 && f.p[4] <= 2147483647.0;
+            if (!ok) return fail(c, PEARL_ERR_ARGUMENT, "invalid FILL operation " + std::to_string(k));
         }
-        if (!ok) return fail(c, PEARL_ERR_ARGUMENT, "invalid FILL operation " + std::to_string(k));
-    }
-    st = read_scalars(c);
-    if (st) return st;
-    const KP& P = c->kp[0];
-    FillOp* d_ops = nullptr;
-    if (count > 0) {
-        HIP_CHECK(c, hipMalloc(&d_ops, static_cast<size_t>(count) * sizeof(FillOp)));
-        const hipError_t e =
-            hipMemcpyAsync(d_ops, ops, static_cast<size_t>(count) * sizeof(FillOp), hipMemcpyHostToDevice, c->stream);
-        if (e != hipSuccess) {
-            (void)hipFree(d_ops);
-            return hip_fail(c, e, "hipMemcpyAsync(fill operations)");
+        st = read_scalars(c);
+        if (st) return st;
+        const KP& P = c->kp[0];
+        FillOp* d_ops = nullptr;
+        if (count > 0) {
+            HIP_CHECK(c, hipMalloc(&d_ops, static_cast<size_t>(count) * sizeof(FillOp)));
+            const hipError_t e =
+                hipMemcpyAsync(d_ops, ops, static_cast<size_t>(count) * sizeof(FillOp), hipMemcpyHostToDevice, c->stream);
+            if (e != hipSuccess) {
+                (void)hipFree(d_ops);
+                return hip_fail(c, e, "hipMemcpyAsync(fill operations)");
+            }
         }
+        PEARL_LAUNCH(c, k_fill, dim3(static_cast<unsigned>(P.nb)), c->B, 0, d_ops, count);
+        hipError_t e = hipGetLastError();
+        const hipError_t e2 = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = e2;
+        if (d_ops) (void)hipFree(d_ops);
+        if (e != hipSuccess) return hip_fail(c, e, "k_fill");
+        // the energy and mass of the previous phases no longer hold
+        DevScal& h = c->hS[0];
+        h.e0_valid = 0;
+        h.m0_valid = 0;
+        return write_scalars(c);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    PEARL_LAUNCH(c, k_fill, dim3(static_cast<unsigned>(P.nb)), c->B, 0, d_ops, count);
-    hipError_t e = hipGetLastError();
-    const hipError_t e2 = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess) e = e2;
-    if (d_ops) (void)hipFree(d_ops);
-    if (e != hipSuccess) return hip_fail(c, e, "k_fill");
-    // the energy and mass of the previous phases no longer hold
-    DevScal& h = c->hS[0];
-    h.e0_valid = 0;
-    h.m0_valid = 0;
-    return write_scalars(c);
 }
 
 int pearl_init_composition(pearl_ctx* c, int mode, double value, double* mu_out, double* carbon_out,
                            int* valid_out) {
-    int st = ctx_check(c, 0);
-    if (st) return st;
-    if ((mode != PEARL_COMPOSITION_MU && mode != PEARL_COMPOSITION_MEAN) || !std::isfinite(value) || !mu_out ||
-        !carbon_out || !valid_out)
-        return fail(c, PEARL_ERR_ARGUMENT, "invalid composition set-up");
-    st = read_scalars(c);
-    if (st) return st;
-    const KP& P = c->kp[0];
-    const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
-    DevScal& h = c->hS[0];
-    double mu = value;
-    if (mode == PEARL_COMPOSITION_MEAN) {  // :1814-1822
-        PEARL_LAUNCH(c, k_init_sums, grid, c->B, 0);
-        hipLaunchKernelGGL(k_init_sums_fin, one, tb, 0, c->stream, c->B, 0);
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        if ((mode != PEARL_COMPOSITION_MU && mode != PEARL_COMPOSITION_MEAN) || !std::isfinite(value) || !mu_out ||
+            !carbon_out || !valid_out)
+            return fail(c, PEARL_ERR_ARGUMENT, "invalid composition set-up");
         st = read_scalars(c);
         if (st) return st;
-        if (h.in_bad > 0) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
-        mu = (static_cast<double>(P.nx * P.ny) * value - h.in_c0) / h.in_s;
+        const KP& P = c->kp[0];
+        const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
+        DevScal& h = c->hS[0];
+        double mu = value;
+        if (mode == PEARL_COMPOSITION_MEAN) {  // :1814-1822
+            PEARL_LAUNCH(c, k_init_sums, grid, c->B, 0);
+            hipLaunchKernelGGL(k_init_sums_fin, one, tb, 0, c->stream, c->B, 0);
+            st = read_scalars(c);
+            if (st) return st;
+            if (h.in_bad > 0) return fail(c, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+            mu = (static_cast<double>(P.nx * P.ny) * value - h.in_c0) / h.in_s;
+        }
+        PEARL_LAUNCH(c, k_init_carbon, grid, c->B, 0, mu);
+        double valid = 0, mass = 0;
+        st = state_direct(c, 0, true, &valid, &mass, nullptr);
+        if (st) return st;
+        *mu_out = mu;
+        *carbon_out = mass;
+        *valid_out = valid > 0.5 ? 1 : 0;
+        h.e0_valid = 0;
+        h.m0_valid = 0;
+        return write_scalars(c);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
     }
-    PEARL_LAUNCH(c, k_init_carbon, grid, c->B, 0, mu);
-    double valid = 0, mass = 0;
-    st = state_direct(c, 0, true, &valid, &mass, nullptr);
-    if (st) return st;
-    *mu_out = mu;
-    *carbon_out = mass;
-    *valid_out = valid > 0.5 ? 1 : 0;
-    h.e0_valid = 0;
-    h.m0_valid = 0;
-    return write_scalars(c);
 }
 
 int pearl_reduced_energy(pearl_ctx* c, double* energy) {
