@@ -754,6 +754,9 @@ def main():
             if tr:
                 roof["traffic"] = round(tr["traffic_bytes"])
                 roof["traffic_source"] = os.path.relpath(measured, ROOT)
+                # measured HBM rate (counter bytes per launch / launch time)
+                roof["traffic_GBps"] = round(tr["traffic_bytes"] / (roof["avg_launch_ms"] * 1e-3) / 1e9, 1)
+                roof["traffic_frac"] = round(roof["traffic_GBps"] / HBM_PEAK_GBS, 4)
     total_bytes = sum(v["bytes"] for v in counted.values())
     out = {
         "metric": "Mcell-steps/s", "value": round(mcs, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
