@@ -5,6 +5,8 @@ a state produced by the reference's initialize(), and the reference's
 outputs for single calls and short trajectories from that state.
 
 Run from the repository root:  python tests/golden/make_golden.py
+(--only name1,name2 writes just those fixtures; --full adds the whole
+smoke-case trajectory).
 """
 from __future__ import annotations
 
@@ -70,34 +72,60 @@ def full_run(name, path, end_time):
     print(name, "written:", st, "t =", sc1[0], "rejected =", sc1[8])
 
 
+def want(name):
+    """--only a,b writes just those fixtures (the others stay as they are)."""
+    for k, arg in enumerate(sys.argv):
+        if arg == "--only":
+            return name in sys.argv[k + 1].split(",")
+    return True
+
+
 def main():
     tmp = tempfile.mkdtemp()
     # Small colony: 2 lamella pairs, 20 smoothing steps, 40 steps.
     p = inputs.colony_input(os.path.join(tmp, "small.in"), 48, 40, 2, 10, 1.5e-9,
                             overrides={"smooth_steps": 20, "smooth_min_steps": 20})
-    case("small_48x40", p, 40, 1.5e-9, 1.5e-11)
+    if want("small_48x40"):
+        case("small_48x40", p, 40, 1.5e-9, 1.5e-11)
     # Phase-only mode on the same geometry.
     p = inputs.colony_input(os.path.join(tmp, "phase_only.in"), 48, 40, 2, 10, 1.5e-9,
                             overrides={"smooth_steps": 20, "smooth_min_steps": 20, "mode": "phase_only"})
-    case("phase_only_48x40", p, 20, 1.5e-9, 1.5e-11)
+    if want("phase_only_48x40"):
+        case("phase_only_48x40", p, 20, 1.5e-9, 1.5e-11)
     # Moving window: a tall product strip so the front sits above the trigger.
     p = inputs.colony_input(os.path.join(tmp, "window.in"), 32, 40, 2, 30, 1.5e-9,
                             overrides={"smooth_steps": 10, "smooth_min_steps": 10, "moving_window": 1})
-    case("window_32x40", p, 5, 1.5e-9, 1.5e-11, window=(0.5 * 40 * 2.79e-9, 0.3 * 40 * 2.79e-9,
+    if want("window_32x40"):
+        case("window_32x40", p, 5, 1.5e-9, 1.5e-11, window=(0.5 * 40 * 2.79e-9, 0.3 * 40 * 2.79e-9,
                                                          0.034458832049305582))
     # Moving window that shifts: a pure-alpha strip (no gamma in the removed rows).
     p = inputs.write_input(os.path.join(tmp, "window_alpha.in"), 32, 40, 1.5e-9, 1.5e-8,
                            [("FILLCUBE", "{0,0,0,0,31,29,0}")],
                            overrides={"smooth_steps": 10, "smooth_min_steps": 10, "moving_window": 1})
-    case("window_alpha_32x40", p, 5, 1.5e-9, 1.5e-11, window=(0.5 * 40 * 2.79e-9, 0.3 * 40 * 2.79e-9,
+    if want("window_alpha_32x40"):
+        case("window_alpha_32x40", p, 5, 1.5e-9, 1.5e-11, window=(0.5 * 40 * 2.79e-9, 0.3 * 40 * 2.79e-9,
                                                                0.034458832049305582))
     # The 256x128 lamellar smoke case: 4 pairs of 8 theta + 56 alpha columns,
     # rows 0-15, full 100-step smoothing; 20 steps.
-    p = inputs.colony_input(os.path.join(tmp, "cfg0.in"), 256, 128, 4, 16, 1.5e-8, output_dt=2.5e-8)
-    case("colony_256x128", p, 20, 1.5e-8, 1.5e-11)
+    p_cfg0 = inputs.colony_input(os.path.join(tmp, "cfg0.in"), 256, 128, 4, 16, 1.5e-8, output_dt=2.5e-8)
+    if want("colony_256x128"):
+        case("colony_256x128", p_cfg0, 20, 1.5e-8, 1.5e-11)
+    # Edge geometries: the smallest grid the reference accepts (3x2: west and
+    # east neighbours are the same cell's two others, one ragged 2-row tile)
+    # and a width one past a tile (257: a second tile column of one cell)
+    # with a ragged last tile row (9 rows).
+    dy = 2.79e-9
+    p = inputs.write_input(os.path.join(tmp, "tiny.in"), 3, 2, 1.5e-9, 1.5e-8, [],
+                           overrides={"initialization": "lamella", "theta_columns": 1, "seed_height": repr(dy)})
+    if want("tiny_3x2"):
+        case("tiny_3x2", p, 20, 1.5e-9, 1.5e-11)
+    p = inputs.colony_input(os.path.join(tmp, "ragged.in"), 257, 9, 2, 4, 1.5e-9,
+                            overrides={"smooth_steps": 10, "smooth_min_steps": 10})
+    if want("ragged_257x9"):
+        case("ragged_257x9", p, 20, 1.5e-9, 1.5e-11)
     if "--full" in sys.argv:
         # Whole smoke-case trajectory to 0.015 us (about a minute of CPU).
-        full_run("colony_256x128_full", p, 1.5e-8)
+        full_run("colony_256x128_full", p_cfg0, 1.5e-8)
 
 
 if __name__ == "__main__":

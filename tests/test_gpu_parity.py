@@ -19,7 +19,7 @@ from conftest import load_golden
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["small_48x40", "phase_only_48x40", "window_alpha_32x40", "colony_256x128"]
+CASES = ["small_48x40", "phase_only_48x40", "window_alpha_32x40", "colony_256x128", "tiny_3x2", "ragged_257x9"]
 MU_TOL = 1e-5
 PHI_TOL = 1e-10
 

@@ -6,7 +6,8 @@ import pytest
 
 from conftest import load_golden
 
-CASES = ["small_48x40", "phase_only_48x40", "window_32x40", "window_alpha_32x40", "colony_256x128"]
+CASES = ["small_48x40", "phase_only_48x40", "window_32x40", "window_alpha_32x40", "colony_256x128", "tiny_3x2",
+         "ragged_257x9"]
 
 SHIFT_ERRORS = {"unsafe shift: removed strip contains gamma": 6, "invalid shift": 8}
 
