@@ -1438,6 +1438,9 @@ int main(int argc, char** argv) {
         Config config = read_config(input);
         if (stop_steps == 0 || stop_steps < -1) throw std::invalid_argument("max-steps must be positive");
         if (config.nx < 3 || config.ny < 2 || config.dx <= 0 || config.dy <= 0) throw std::invalid_argument("invalid grid");
+        // no reference counterpart: the device context's size limit (pearl_params)
+        if (3 * static_cast<long long>(config.nx) * config.ny >= (1LL << 31) - (1LL << 20))
+            throw std::invalid_argument("grid too large for one device context (3*nx*ny must stay below 2^31 - 2^20)");
         if (px != 1 || py != 1) throw std::invalid_argument("serial build requires px=py=1");
         {
             bool ok = true;

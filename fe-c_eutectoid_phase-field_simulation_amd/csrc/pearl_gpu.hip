@@ -431,7 +431,9 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
 bool params_ok(const pearl_params* p) {
     if (!p) return false;
     if (p->nx < 3 || p->ny < 2 || !(p->dx > 0) || !(p->dy > 0)) return false;
-    if (static_cast<long long>(p->nx) * p->ny >= (1LL << 31) - (1LL << 20)) return false;
+    // the kernels index the three phase planes of a domain as a*n + i in
+    // 32 bits: 3n stays below 2^31 (with room for the neighbour offsets)
+    if (3 * static_cast<long long>(p->nx) * p->ny >= (1LL << 31) - (1LL << 20)) return false;
     for (int i = 0; i < 3; ++i)
         if (!(p->A[i] > 0) || !std::isfinite(p->A[i]) || !(p->tau[i] > 0)) return false;
     if (!(p->eps > 0) || !(p->sigma > 0) || !(p->Vm > 0) || p->triple < 0) return false;

@@ -61,7 +61,8 @@ extern "C" {
 /* Model and numerical parameters: the reference's Domain (:617), Thermo
  * (:76-91), PhaseParameters (:521-532) and Controls (:767-773). */
 typedef struct {
-    int nx, ny;                        /* global cells; x periodic, y zero flux */
+    int nx, ny;                        /* global cells; x periodic, y zero flux; nx >= 3, ny >= 2,
+                                          3*nx*ny < 2^31 - 2^20 */
     double dx, dy;                     /* m */
     double A[3], B[3], C[3], D[3];     /* f_p(c)=A c^2+B c+C [J/mol]; D_p [m^2/s] */
     double eps, sigma, triple;         /* interface parameters */

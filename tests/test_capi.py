@@ -85,3 +85,19 @@ def test_argument_checks_before_any_device_call():
     mu, carbon, valid = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
     assert L.pearl_init_composition(None, 0, 1.0, ctypes.byref(mu), ctypes.byref(carbon),
                                     ctypes.byref(valid)) == pkg.ERR_ARGUMENT
+
+
+def test_domain_size_limit():
+    """A domain whose three phase planes would overflow the kernels' 32-bit
+    plane offsets is rejected before any device call."""
+    pkg = load_pkg()
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import load_golden
+    pd = dict(load_golden("small_48x40")["pd"])
+    h = ctypes.c_void_p()
+    for nx, ny in ((40000, 20000), (715827, 1000)):  # 800 M and 716 M cells
+        pd.update(nx=nx, ny=ny)
+        P = pkg.params_from_dict(pd)
+        assert pkg.lib().pearl_create(ctypes.byref(P), 3.1e4, pkg.FP64, 0, 0, ctypes.byref(h)) == pkg.ERR_ARGUMENT
+        assert not h.value
