@@ -733,6 +733,53 @@ __device__ __forceinline__ void propose_cell(const KP& P, const double* __restri
     for (int a = 0; a < 3; ++a) st_once(out + a * n + i, zp[a]);
 }
 
+// LDS tile of a cell kernel: TY rows of the block's 256 columns, the row
+// above and the row below (clamped at the domain's bottom and top rows, where
+// a missing neighbour is the cell itself), and one column either side
+// (periodic in x).  stage_tile calls put(r, cc, j) for LDS row r, LDS column
+// cc and cell j; each thread stages its own column, threads 0 and 255 the
+// two halo columns.  The tile is formed once per block, so a neighbour's
+// value is neither loaded again nor recomputed by the cells around it.
+template <int TY>
+struct TileLds {
+    double a[TY + 2][TPB + 2], b[TY + 2][TPB + 2];
+};
+
+template <int TY, class F>
+__device__ __forceinline__ void stage_tile(const KP& P, const Col& col, F&& put) {
+    const int t = static_cast<int>(threadIdx.x);
+    const int x0 = col.x - t;
+    auto one = [&](const int cc, int gx) {
+        gx = gx < 0 ? gx + P.nx : (gx >= P.nx ? gx % P.nx : gx);
+#pragma unroll
+        for (int r = 0; r < TY + 2; ++r) {
+            int y = col.y0 - 1 + r;
+            y = y < 0 ? 0 : (y > P.ny - 1 ? P.ny - 1 : y);
+            put(r, cc, y * P.nx + gx);
+        }
+    };
+    one(t + 1, x0 + t);
+    if (t == 0) one(0, x0 - 1);
+    if (t == TPB - 1) one(TPB + 1, x0 + TPB);
+}
+
+// The block's cells in the march order, after staging: body(i, y, r, cc)
+// with LDS row r (south r - 1, north r + 1) and column cc (west cc - 1,
+// east cc + 1).
+template <int TY, class F>
+__device__ __forceinline__ void for_tile(const KP& P, const Col& col, F&& body) {
+    if (!col.on) return;
+    const int cc = static_cast<int>(threadIdx.x) + 1;
+    int i = col.i0;
+#pragma unroll
+    for (int r = 1; r <= TY; ++r) {
+        const int y = col.y0 + r - 1;
+        if (y >= col.y1) break;
+        body(i, y, r, cc);
+        i += P.nx;
+    }
+}
+
 struct Phases {
     double p[3];
 };
@@ -846,44 +893,44 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
-    march<TY, Prep>(
-        P, col,
-        [&](const int j) {
-            Prep q;
-            double p[3];
-            load_phi(phi_t, n, j, p);
-            q.ok = mixture(P, p, q.m);
-            q.k = q.m.K;
-            q.x = mu_old[j] - P.mu_ref;
-            return q;
-        },
-        [&](const int i, const int y, const Prep& s, const Prep& c, const Prep& nn) {
-            if (!c.ok) v[3] = 1.0;
-            const double ic = composition(P.mu_ref, c.m);
-            const double b = ld_once(c_old + i) - ic;
-            const double ch = c.m.s;
-            st_once(chi + i, ch);
-            st_once(icpt + i, ic);
-            st_once(K + i, c.k);
-            st_once(rhs + i, b);
-            st_once(x + i, c.x);
-            double qw[3], qe[3];
-            load_phi(phi_t, n, i + col.dw, qw);
-            load_phi(phi_t, n, i + col.de, qe);
-            const double xw = mu_old[i + col.dw] - P.mu_ref, xe = mu_old[i + col.de] - P.mu_ref;
-            double wf[4];
-            face_weights(P, y, c.k, mixture_K(P, qw), mixture_K(P, qe), s.k, nn.k, wf);
-            const double ax = apply_vals(ch, dt, wf, c.x, xw, xe, s.x, nn.x);
-            const double d = stored<MX>(ch + dt * (wf[0] + wf[1] + wf[2] + wf[3]));
-            const double rr = b - ax;
-            const double zz = stored<MX>(rr / d);
-            st_once(diag + i, static_cast<T>(d));
-            st_once(r + i, rr);
-            st_once(z + i, static_cast<T>(zz));
-            v[0] += b * b;
-            v[1] += rr * zz;
-            v[2] += rr * rr;
-        });
+    // LDS tile: x = mu_old - mu_ref and K = mixture's K of the trial phases,
+    // for the tile and its halo (each formed once per block)
+    __shared__ TileLds<TY> L;
+    stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
+        double p[3];
+        load_phi(phi_t, n, j, p);
+        L.a[r][cc] = mu_old[j] - P.mu_ref;
+        L.b[r][cc] = mixture_K(P, p);
+    });
+    __syncthreads();
+    for_tile<TY>(P, col, [&](const int i, const int y, const int row, const int cc) {
+        double p[3];
+        load_phi(phi_t, n, i, p);
+        Mix m;
+        if (!mixture(P, p, m)) v[3] = 1.0;
+        const double ic = composition(P.mu_ref, m);
+        const double b = ld_once(c_old + i) - ic;
+        const double ch = m.s;
+        const double kc = L.b[row][cc], xc = L.a[row][cc];
+        st_once(chi + i, ch);
+        st_once(icpt + i, ic);
+        st_once(K + i, kc);
+        st_once(rhs + i, b);
+        st_once(x + i, xc);
+        double wf[4];
+        face_weights(P, y, kc, L.b[row][cc - 1], L.b[row][cc + 1], L.b[row - 1][cc], L.b[row + 1][cc], wf);
+        const double ax =
+            apply_vals(ch, dt, wf, xc, L.a[row][cc - 1], L.a[row][cc + 1], L.a[row - 1][cc], L.a[row + 1][cc]);
+        const double d = stored<MX>(ch + dt * (wf[0] + wf[1] + wf[2] + wf[3]));
+        const double rr = b - ax;
+        const double zz = stored<MX>(rr / d);
+        st_once(diag + i, static_cast<T>(d));
+        st_once(r + i, rr);
+        st_once(z + i, static_cast<T>(zz));
+        v[0] += b * b;
+        v[1] += rr * zz;
+        v[2] += rr * rr;
+    });
     block_partial<4, (1u << 6)>(v, B.part + P.poff, P.nb, w.lb);
 }
 
@@ -913,6 +960,7 @@ __global__ __launch_bounds__(TPB) void k_prepare_fin(Bufs B) {
 struct DirK {
     double p, k;
 };
+
 // Solution and diffusivity of one cell.
 struct SolK {
     double x, k;
@@ -937,6 +985,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const double* __restrict__ K = B.K + P.off;
     T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
     const Col col = column<TY>(P, w.lb, P.ny);
+    __shared__ TileLds<TY> L;  // a: the stencil variable, b: K
     if (state == CG_ITER) {
         record(B, slot, w.lb, 1);
         const int pi = S->pidx;
@@ -945,28 +994,29 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         double* __restrict__ Ap = B.Ap + P.off;
         const double beta = S->beta;
         double v[1] = {0.0};
-        // The restart flag is uniform; one march per value keeps the p_old
-        // loads free of branches (and of the waits a branch forces).
-        auto run = [&](auto fresh) {
-            // the direction as it is stored; A p is formed from exactly these values
-            auto dir = [&](const int j) {
+        // the direction as it is stored; A p is formed from exactly these
+        // values.  The restart flag is uniform: one staging loop per value
+        // keeps the p_old loads free of branches.
+        auto stage = [&](auto fresh) {
+            stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
                 const double zj = z[j];
-                return stored<MX>(decltype(fresh)::value ? zj : zj + beta * static_cast<double>(po[j]));
-            };
-            march<TY, DirK>(
-                P, col, [&](const int j) { return DirK{dir(j), K[j]}; },
-                [&](const int i, const int y, const DirK& s, const DirK& c, const DirK& nn) {
-                    const int iw = i + col.dw, ie = i + col.de;
-                    double wf[4];
-                    face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
-                    const double a = apply_vals(ld_once(chi + i), dt, wf, c.p, dir(iw), dir(ie), s.p, nn.p);
-                    st_once(pn + i, static_cast<T>(c.p));
-                    st_once(Ap + i, a);
-                    v[0] += c.p * a;
-                });
+                L.a[r][cc] = stored<MX>(decltype(fresh)::value ? zj : zj + beta * static_cast<double>(po[j]));
+                L.b[r][cc] = K[j];
+            });
         };
-        if (S->restart != 0) run(std::true_type{});
-        else run(std::false_type{});
+        if (S->restart != 0) stage(std::true_type{});
+        else stage(std::false_type{});
+        __syncthreads();
+        for_tile<TY>(P, col, [&](const int i, const int y, const int r, const int cc) {
+            double wf[4];
+            face_weights(P, y, L.b[r][cc], L.b[r][cc - 1], L.b[r][cc + 1], L.b[r - 1][cc], L.b[r + 1][cc], wf);
+            const double c = L.a[r][cc];
+            const double a = apply_vals(ld_once(chi + i), dt, wf, c, L.a[r][cc - 1], L.a[r][cc + 1], L.a[r - 1][cc],
+                                        L.a[r + 1][cc]);
+            st_once(pn + i, static_cast<T>(c));
+            st_once(Ap + i, a);
+            v[0] += c * a;
+        });
         block_partial<1, 0u>(v, B.part + P.poff, P.nb, w.lb);
     } else if (state == CG_TRUE) {
         record(B, slot, w.lb, 2);
@@ -975,20 +1025,22 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         const T* __restrict__ diag = reinterpret_cast<const T*>(B.diag) + P.off;
         double* __restrict__ r = B.r + P.off;
         double v[2] = {0.0, 0.0};
-        march<TY, SolK>(
-            P, col, [&](const int j) { return SolK{x[j], K[j]}; },
-            [&](const int i, const int y, const SolK& s, const SolK& c, const SolK& nn) {
-                const int iw = i + col.dw, ie = i + col.de;
-                double wf[4];
-                face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
-                const double rr =
-                    ld_once(rhs + i) - apply_vals(ld_once(chi + i), dt, wf, c.x, x[iw], x[ie], s.x, nn.x);
-                const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
-                st_once(r + i, rr);
-                st_once(z + i, static_cast<T>(zz));
-                v[0] += rr * rr;
-                v[1] += rr * zz;
-            });
+        stage_tile<TY>(P, col, [&](const int rr, const int cc, const int j) {
+            L.a[rr][cc] = x[j];
+            L.b[rr][cc] = K[j];
+        });
+        __syncthreads();
+        for_tile<TY>(P, col, [&](const int i, const int y, const int rw, const int cc) {
+            double wf[4];
+            face_weights(P, y, L.b[rw][cc], L.b[rw][cc - 1], L.b[rw][cc + 1], L.b[rw - 1][cc], L.b[rw + 1][cc], wf);
+            const double rr = ld_once(rhs + i) - apply_vals(ld_once(chi + i), dt, wf, L.a[rw][cc], L.a[rw][cc - 1],
+                                                            L.a[rw][cc + 1], L.a[rw - 1][cc], L.a[rw + 1][cc]);
+            const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
+            st_once(r + i, rr);
+            st_once(z + i, static_cast<T>(zz));
+            v[0] += rr * rr;
+            v[1] += rr * zz;
+        });
         block_partial<2, 0u>(v, B.part + P.poff, P.nb, w.lb);
     }
 }
@@ -1111,22 +1163,29 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     double v[4] = {0.0, 1.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
     const Col col = column<TY>(P, w.lb, P.ny);
-    march<TY, Fin>(
+    // LDS tile: x and K of the tile and its halo; the trial phases march
+    // down the column in registers (the energy needs east and north)
+    __shared__ TileLds<TY> L;
+    stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
+        L.a[r][cc] = x[j];
+        L.b[r][cc] = K[j];
+    });
+    __syncthreads();
+    const int cc = static_cast<int>(threadIdx.x) + 1;
+    march<TY, Phases>(
         P, col,
         [&](const int j) {
-            Fin q;
-            q.x = x[j];
-            q.k = K[j];
+            Phases q;
             load_phi(phi_t, n, j, q.p);
             return q;
         },
-        [&](const int i, const int y, const Fin& s, const Fin& c, const Fin& nn) {
-            const int iw = i + col.dw, ie = i + col.de;
+        [&](const int i, const int y, const Phases&, const Phases& c, const Phases& nn) {
+            const int r = y - col.y0 + 1;
             double wf[4];
-            face_weights(P, y, c.k, K[iw], K[ie], s.k, nn.k, wf);
-            const double xi = c.x;
-            const double change =
-                dt * (wf[0] * (x[iw] - xi) + wf[1] * (x[ie] - xi) + wf[2] * (s.x - xi) + wf[3] * (nn.x - xi));
+            face_weights(P, y, L.b[r][cc], L.b[r][cc - 1], L.b[r][cc + 1], L.b[r - 1][cc], L.b[r + 1][cc], wf);
+            const double xi = L.a[r][cc];
+            const double change = dt * (wf[0] * (L.a[r][cc - 1] - xi) + wf[1] * (L.a[r][cc + 1] - xi) +
+                                        wf[2] * (L.a[r - 1][cc] - xi) + wf[3] * (L.a[r + 1][cc] - xi));
             const double cn = ld_once(c_old + i) + change;
             const double ic = ld_once(icpt + i), ch = ld_once(chi + i);
             v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
@@ -1134,7 +1193,7 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
             st_once(c_new + i, cn);
             st_once(mu_new + i, mn);
             double qe[3];
-            load_phi(phi_t, n, ie, qe);
+            load_phi(phi_t, n, i + col.de, qe);
             double e;
             accept_vals(P, c.p, qe, nn.p, y != P.ny - 1, cn, mn, good, e, mix_ok);
             v[2] += cn;
