@@ -1115,7 +1115,9 @@ int copy_state(pearl_ctx* c, double* dst_phi, double* dst_c, double* dst_mu, con
 
 // Adjoint PCG solve A lam = gx (A of the trial phases, dt of the step),
 // lam left in B.x.  Reuses the forward PCG kernels and their controller.
-int adjoint_solve(pearl_ctx* c, const Adj& A) {
+// `slots`: PCG slots enqueued per synchronisation; raised to the solve's
+// iterations + 1 when a solve needs more (the next solve then fits).
+int adjoint_solve(pearl_ctx* c, const Adj& A, int& slots) {
     DevScal& h = c->hS[0];
     h.phase = PH_SOLVE;
     h.cg_state = CG_IDLE;
@@ -1134,7 +1136,7 @@ int adjoint_solve(pearl_ctx* c, const Adj& A) {
     // PCG slots, then one synchronisation to see whether the solve is done
     // (slots after convergence are predicated off)
     for (int guard = 0; guard < 10000; ++guard) {
-        for (int k = 0; k < c->cg_slots; ++k) {
+        for (int k = 0; k < slots; ++k) {
             PEARL_LAUNCH_P(c, k_cg_a, grid, c->B, -1);
             hipLaunchKernelGGL(k_cg_a_fin, one, tb, 0, c->stream, c->B);
             PEARL_LAUNCH_P(c, k_cg_b, grid, c->B, -1);
@@ -1145,6 +1147,7 @@ int adjoint_solve(pearl_ctx* c, const Adj& A) {
         if (h.cg_state == CG_OK) break;
         if (h.cg_state == CG_FAIL) return fail(c, PEARL_ERR_ARGUMENT, "adjoint: the linear solve did not converge");
     }
+    slots = std::min(24, std::max(slots, h.cg_it + 1));
     h.phase = PH_IDLE;
     return write_scalars(c);
 }
@@ -1237,6 +1240,7 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
         // reverse sweep, one checkpointed segment at a time
         const dim3 grid(static_cast<unsigned>(P.nb)), one(1), tb(TPB);
         int cur = 1;  // G[cur] holds the gradient w.r.t. the later state
+        int slots = c->cg_slots;
         for (int sgi = segs - 1; sgi >= 0; --sgi) {
             const int s0 = sgi * every, len = std::min(every, steps - s0);
             if ((st = load(ck + state * sgi))) return st;
@@ -1249,7 +1253,7 @@ int pearl_adjoint(pearl_ctx* c, int steps, const double* dts, int every, const d
                 double* gout = G[cur ^ 1];
                 Adj A{ph(s_n), mm(s_n), ph(s_n1), mm(s_n1), ph(gin), cc(gin), mm(gin), ph(gout), cc(gout), mm(gout),
                       gphit, vbuf, theta, dts[s0 + k]};
-                if ((st = adjoint_solve(c, A))) return st;
+                if ((st = adjoint_solve(c, A, slots))) return st;
                 PEARL_LAUNCH(c, k_adj_diffgrad, grid, c->B, A);
                 hipLaunchKernelGGL(k_adj_diffgrad_fin, one, tb, 0, c->stream, c->B, A);
                 PEARL_LAUNCH(c, k_adj_propose_a, grid, c->B, A);
