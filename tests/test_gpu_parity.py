@@ -51,7 +51,7 @@ def test_propose_phi_bitwise_vs_reference(pkg, name, k):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("k", [1, 2])
+@pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
 def test_implicit_diffusion(pkg, O, name, k):
     g = load_golden(name)
     if g["pd"]["phase_only"]:
@@ -119,6 +119,28 @@ def test_trajectory(pkg, O, name):
     assert np.abs(phi - g["phi1"]).max() <= PHI_TOL
     assert np.abs(mu - g["mu1"]).max() <= MU_TOL
     assert sc.to_list()[6:] == list(g["sc1"][6:9])
+
+
+@pytest.mark.parametrize("name", ["colony_256x128", "ragged_257x9", "tiny_3x2"])
+@pytest.mark.parametrize("k", [8, 16])
+def test_trajectory_tall_tiles(pkg, O, name, k):
+    """Whole steps with tiles of 8 and 16 rows (the LDS tiles of prepare,
+    cg_apply, the true residual and finalize hold k + 2 rows): bitwise equal
+    to the oracle in the same blocked order."""
+    g = load_golden(name)
+    steps, end = int(g["steps"]), float(g["end_time"])
+    say("trajectory", name, "k", k)
+    with make(pkg, g, k) as s:
+        infos = [s.advance(end) for _ in range(steps)]
+        phi, c, mu, sc = s.get_state()
+    P = blocked_params(O, g, k)
+    S = O.OracleState(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+    for kk in range(steps):
+        rc, oi = O.advance(P, S, float(g["sc0"][9]), end)
+        assert rc == 0
+        assert infos[kk].cg_iterations == oi.cg_iterations and infos[kk].accepted_dt == oi.accepted_dt, kk
+    assert np.array_equal(phi, S.phi) and np.array_equal(c, S.carbon) and np.array_equal(mu, S.mu)
+    assert sc.to_list() == S.scalars()
 
 
 @pytest.mark.parametrize("name", ["window_32x40", "window_alpha_32x40"])
