@@ -210,7 +210,8 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
             load_phi(A.phi0, n, i + col.de, qe);
             const double m = A.mu0[i];
             double force[3], mean, scale, z[3];
-            propose_z(P, y, p, qw, qe, ps.p, pn.p, m, dt, force, mean, scale, z);
+            double north[3];
+            propose_z(P, y, p, qw, qe, ps.p, pn.p, m, dt, force, mean, scale, z, nullptr, north);
             double gt[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) gt[a] = A.gphit[a * n + i];

@@ -668,10 +668,16 @@ __global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
 // the step scale and z = phi - scale*(force - mean), in the reference's
 // operation order.  p and its west, east, south, north neighbours qw, qe,
 // qs, qn (the south/north terms are skipped at the y edges).
+// The face term w of a pair on a cell's south face is exactly the negation
+// of the term on its southern neighbour's north face (the difference, the
+// product and the constant division are all sign-symmetric in IEEE
+// arithmetic), so a column march passes the north terms down: `south`, when
+// given, holds the southern neighbour's north terms, and `north` receives
+// this cell's.
 __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[3], const double (&qw)[3],
                                           const double (&qe)[3], const double (&qs)[3], const double (&qn)[3],
                                           double m, double dt, double (&force)[3], double& mean, double& scale,
-                                          double (&z)[3]) {
+                                          double (&z)[3], const double* south, double (&north)[3]) {
     const double psi0 = psi(P, 2, m);
     const double dtr[3] = {2.0 * p[1] * p[2], 2.0 * p[0] * p[2], 2.0 * p[0] * p[1]};
     double dpsi[3];
@@ -696,7 +702,10 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = a + 1; b < 3; ++b) {
-                const double wv = div_const(P.fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                const double wv = (dir == 2 && south)
+                                      ? -south[pair_of(a, b)]
+                                      : div_const(P.fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                if (dir == 3) north[pair_of(a, b)] = wv;
                 force[a] += wv * q[b];
                 force[b] -= wv * q[a];
             }
@@ -713,12 +722,12 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
 __device__ __forceinline__ void propose_cell(const KP& P, const double* __restrict__ phi, int n, int i, int y,
                                              const Col& col, const double (&ps)[3], const double (&p)[3],
                                              const double (&pn)[3], double m, double dt, double* __restrict__ out,
-                                             double (&v)[2]) {
+                                             double (&v)[2], const double* south, double (&north)[3]) {
     double qw[3], qe[3];
     load_phi(phi, n, i + col.dw, qw);
     load_phi(phi, n, i + col.de, qe);
     double force[3], mean, scale, z[3];
-    propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z);
+    propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north);
     if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
         v[0] = __builtin_inf();
 #pragma unroll
@@ -799,6 +808,7 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     double* __restrict__ out = B.phi[cur ^ 1] + 3 * P.off;
     const double dt = S->dt;
     double v[2] = {0.0, 0.0};
+    double carry[3] = {0.0, 0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     march<TY, Phases>(
         P, col,
@@ -808,7 +818,12 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
             return q;
         },
         [&](const int i, const int y, const Phases& s, const Phases& c, const Phases& nn) {
-            propose_cell(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v);
+            // the previous cell of the march (y - 1) computed this cell's south terms
+            double north[3] = {0.0, 0.0, 0.0};
+            propose_cell(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
+                         y > col.y0 ? carry : nullptr, north);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) carry[q] = north[q];
         });
     block_partial<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
