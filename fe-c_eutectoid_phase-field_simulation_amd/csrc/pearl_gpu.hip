@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pearl_adjoint.hpp"
@@ -150,6 +151,26 @@ int hip_fail(pearl_ctx* c, hipError_t e, const char* where) {
         else PEARL_LAUNCH_K(ctx, kname, false, grid, __VA_ARGS__);          \
     } while (0)
 
+// cg_update runs TPB * cgb_split<k> threads per block (two per column).
+void launch_cg_b(pearl_ctx* c, dim3 grid, const Bufs& B, int slot) {
+    auto go = [&](auto ty, auto mx) {
+        constexpr int TY = decltype(ty)::value;
+        constexpr bool MX = decltype(mx)::value;
+        hipLaunchKernelGGL((k_cg_b<TY, MX>), grid, dim3(TPB * cgb_split<TY>()), 0, c->stream, B, slot);
+    };
+    auto by_k = [&](auto mx) {
+        switch (c->k) {
+            case 1: go(std::integral_constant<int, 1>{}, mx); break;
+            case 2: go(std::integral_constant<int, 2>{}, mx); break;
+            case 4: go(std::integral_constant<int, 4>{}, mx); break;
+            case 16: go(std::integral_constant<int, 16>{}, mx); break;
+            default: go(std::integral_constant<int, 8>{}, mx); break;
+        }
+    };
+    if (c->mx) by_k(std::true_type{});
+    else by_k(std::false_type{});
+}
+
 hipEvent_t get_event(pearl_ctx* c) {
     if (!c->free_events.empty()) {
         hipEvent_t e = c->free_events.back();
@@ -208,7 +229,7 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     for (int j = 0; j < slots; ++j) {
         ev(PEARL_KERNEL_CG_APPLY, [&](int s) { PEARL_LAUNCH_P(c, k_cg_a, g, B, s); },
            [&] { hipLaunchKernelGGL(k_cg_a_fin, gf, tb, 0, st, B); });
-        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { PEARL_LAUNCH_P(c, k_cg_b, g, B, s); },
+        ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { launch_cg_b(c, g, B, s); },
            [&] { hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B); });
     }
     ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, B, s); },
@@ -1139,7 +1160,7 @@ int adjoint_solve(pearl_ctx* c, const Adj& A, int& slots) {
         for (int k = 0; k < slots; ++k) {
             PEARL_LAUNCH_P(c, k_cg_a, grid, c->B, -1);
             hipLaunchKernelGGL(k_cg_a_fin, one, tb, 0, c->stream, c->B);
-            PEARL_LAUNCH_P(c, k_cg_b, grid, c->B, -1);
+            launch_cg_b(c, grid, c->B, -1);
             hipLaunchKernelGGL(k_cg_b_fin, one, tb, 0, c->stream, c->B);
         }
         st = read_scalars(c);

@@ -173,14 +173,18 @@ __device__ __forceinline__ void wave_tree(double (&v)[NQ]) {
 }
 
 // Stage 2 inside a cell kernel: the block's partials (thread 0 writes them).
+// Blocks larger than TPB (cg_update's) hold the values in their first TPB
+// threads; the other waves only take part in the barrier.
 template <int NQ, unsigned OPS>
 __device__ __forceinline__ void block_partial(double (&v)[NQ], double* __restrict__ part, int nb, int lb) {
     __shared__ double sh[4][NQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    wave_tree<NQ, OPS>(v);
-    if (lane == 0) {
+    if (threadIdx.x < TPB) {
+        wave_tree<NQ, OPS>(v);
+        if (lane == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) sh[w][q] = v[q];
+            for (int q = 0; q < NQ; ++q) sh[w][q] = v[q];
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -479,10 +483,10 @@ struct Col {
 };
 
 template <int TY>
-__device__ __forceinline__ Col column(const KP& P, int lb, int rows) {
+__device__ __forceinline__ Col column(const KP& P, int lb, int rows, int t = -1) {
     Col c;
     const int tx = lb % P.tiles_x, ty = lb / P.tiles_x;
-    c.x = tx * TPB + static_cast<int>(threadIdx.x);
+    c.x = tx * TPB + (t < 0 ? static_cast<int>(threadIdx.x) : t);
     c.y0 = ty * TY;
     c.y1 = c.y0 + TY < rows ? c.y0 + TY : rows;
     c.on = c.x < P.nx;
@@ -1100,8 +1104,17 @@ __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
 // PCG kernel B, after a step of kernel A: x += alpha p; r -= alpha Ap;
 // z = r/diag; sums r.z and r.r; the finisher forms beta (:1077) and
 // evaluates the loop head of the next iteration.
+// Threads per column of cg_update.  Its cells are independent, so two
+// threads share a column's rows and twice as many loads are in flight; the
+// fold stays the column's sequential one: the thread of the upper rows hands
+// its products to the thread of the lower rows through LDS, which adds them
+// after its own, in row order.
+template <int TY>
+__host__ __device__ constexpr int cgb_split() { return TY >= 2 ? 2 : 1; }
+
 template <int TY, bool MX>
-__global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
+__global__ __launch_bounds__(TPB * cgb_split<TY>()) void k_cg_b(Bufs B, int slot) {
+    constexpr int SPLIT = cgb_split<TY>(), RH = TY / SPLIT;  // rows per thread
     using T = Coef<MX>;
     const Where w = where(B, -1);
     const KP P = B.kp[w.d];
@@ -1115,20 +1128,62 @@ __global__ __launch_bounds__(TPB) void k_cg_b(Bufs B, int slot) {
     const double* __restrict__ Ap = B.Ap + P.off;
     const T* __restrict__ diag = reinterpret_cast<const T*>(B.diag) + P.off;
     const double alpha = S->alpha;
+    const int t = static_cast<int>(threadIdx.x) % TPB, h = static_cast<int>(threadIdx.x) / TPB;
+    const Col col = column<TY>(P, w.lb, P.ny, t);
+    double prod[RH][2];
+    int cnt = 0;
+    if (col.on) {
+        int i = col.i0 + h * RH * P.nx;
+#pragma unroll
+        for (int k = 0; k < RH; ++k) {
+            const int y = col.y0 + h * RH + k;
+            if (y >= col.y1) break;
+            // nontemporal throughout: measured against cg_apply run alternately
+            // with it, the pair is 9 % faster than with plain accesses (plain
+            // stores leave dirty lines that the next cg_apply pays for)
+            st_once(x + i, ld_once(x + i) + alpha * static_cast<double>(ld_once(p + i)));
+            const double rr = ld_once(r + i) - alpha * ld_once(Ap + i);
+            const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
+            st_once(r + i, rr);
+            st_once(z + i, static_cast<T>(zz));
+            prod[k][0] = rr * zz;
+            prod[k][1] = rr * rr;
+            ++cnt;
+            i += P.nx;
+        }
+    }
     double v[2] = {0.0, 0.0};
-    const Col col = column<TY>(P, w.lb, P.ny);
-    for_column<TY>(P, col, [&](const int i, const int) {
-        // nontemporal throughout: measured against cg_apply run alternately
-        // with it, the pair is 9 % faster than with plain accesses (plain
-        // stores leave dirty lines that the next cg_apply pays for)
-        st_once(x + i, ld_once(x + i) + alpha * static_cast<double>(ld_once(p + i)));
-        const double rr = ld_once(r + i) - alpha * ld_once(Ap + i);
-        const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
-        st_once(r + i, rr);
-        st_once(z + i, static_cast<T>(zz));
-        v[0] += rr * zz;
-        v[1] += rr * rr;
-    });
+    if (h == 0) {
+#pragma unroll
+        for (int k = 0; k < RH; ++k) {
+            if (k >= cnt) break;
+            v[0] += prod[k][0];
+            v[1] += prod[k][1];
+        }
+    }
+    if constexpr (SPLIT == 2) {
+        __shared__ double hand[RH][2][TPB];
+        __shared__ int hcnt[TPB];
+        if (h == 1) {
+#pragma unroll
+            for (int k = 0; k < RH; ++k) {
+                if (k >= cnt) break;
+                hand[k][0][t] = prod[k][0];
+                hand[k][1][t] = prod[k][1];
+            }
+            hcnt[t] = cnt;
+        }
+        __syncthreads();
+        if (h == 0) {
+            const int m = hcnt[t];
+#pragma unroll
+            for (int k = 0; k < RH; ++k) {
+                if (k >= m) break;
+                v[0] += hand[k][0][t];
+                v[1] += hand[k][1][t];
+            }
+        }
+    }
     block_partial<2, 0u>(v, B.part + P.poff, P.nb, w.lb);
 }
 
