@@ -23,7 +23,10 @@ Prints one JSON line:
             kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE and
             WRITE_SIZE scaled by a 1 GiB copy kernel) from the committed
             counter summary of the same workload and tile height, when there
-            is one
+            is one, and traffic_GBps that traffic over the live launch time;
+            copy_ceiling_GBps is a hand-written 1 GiB device copy measured in
+            the same run (pearl_stream_probe), frac_of_copy the dominant
+            kernel's rate against it
   cpu_baseline  the reference solver compiled from source (or the oracle
             restatement when that is absent) on a bounded sample of the same
             workload on this host's cores
