@@ -1612,7 +1612,8 @@ hipError_t time_copies(const probe_vec* a, probe_vec* b, long long n, int reps, 
 extern "C" {
 
 int pearl_stream_probe(int device, long long bytes, int reps, double* gbps) {
-    if (!gbps || bytes <= 0 || bytes % 16 != 0 || bytes / 16 > (static_cast<long long>(1) << 38) || reps < 1)
+    // one thread per vector: the grid's work-items must stay below 2^32
+    if (!gbps || bytes <= 0 || bytes % 16 != 0 || bytes / 16 > (static_cast<long long>(1) << 31) || reps < 1)
         return PEARL_ERR_ARGUMENT;
     *gbps = 0;
     if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;

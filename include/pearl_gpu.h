@@ -327,12 +327,12 @@ int pearl_counters(const pearl_ctx* ctx, long long* attempts, long long* rejecte
                    long long* cg_iterations);
 
 /* Measurement utility, no reference counterpart: the bandwidth of a plain
- * device copy of `bytes` bytes (a multiple of 16) on HIP device `device`,
- * by a hand-written kernel that moves one 16-byte vector per thread, with
- * and without the nontemporal hint (the faster counts).  `reps` timed
- * copies after three warm-up copies, HIP events on a stream of its own:
- * *gbps = 2 * bytes * reps / seconds / 1e9 (each copy reads and writes
- * `bytes`).  The measured ceiling reported beside the vendor peak. */
+ * device copy of `bytes` bytes (a multiple of 16, at most 32 GiB) on HIP
+ * device `device`, by a hand-written kernel that moves one 16-byte vector
+ * per thread, with and without the nontemporal hint (the faster counts).
+ * `reps` timed copies after three warm-up copies, HIP events on a stream of
+ * its own: *gbps = 2 * bytes * reps / seconds / 1e9 (each copy reads and
+ * writes `bytes`).  The measured ceiling reported beside the vendor peak. */
 int pearl_stream_probe(int device, long long bytes, int reps, double* gbps);
 
 #ifdef __cplusplus

@@ -77,7 +77,7 @@ def test_argument_checks_before_any_device_call():
     pkg = load_pkg()
     L = pkg.lib()
     g = ctypes.c_double(1.0)
-    for nbytes, reps in ((15, 1), (0, 1), (-16, 1), (1 << 20, 0)):
+    for nbytes, reps in ((15, 1), (0, 1), (-16, 1), (1 << 20, 0), ((1 << 35) + 16, 1)):
         assert L.pearl_stream_probe(0, nbytes, reps, ctypes.byref(g)) == pkg.ERR_ARGUMENT, (nbytes, reps)
     assert L.pearl_stream_probe(0, 1 << 20, 1, None) == pkg.ERR_ARGUMENT
     # a null context is an argument error for every context call
