@@ -606,7 +606,9 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
             P.poff = poff;
             off += (static_cast<long long>(P.n) + 63) / 64 * 64;
             boff += P.nb;
-            if (boff > 2147483647LL) return bail0(PEARL_ERR_ARGUMENT);  // one launch grid holds every block
+            // one launch grid holds every block, of up to 2 * TPB work-items
+            // each, and a grid's work-items must stay below 2^32
+            if (boff > 4294967295LL / (2 * TPB)) return bail0(PEARL_ERR_ARGUMENT);
             poff += 8LL * P.nb + 8;
             blk2dom.insert(blk2dom.end(), P.nb, d);
             c->kp.push_back(P);

@@ -310,7 +310,8 @@ const char* pearl_kernel_name(int kernel);
  * cg_total (accumulated), cg_peak, last and status have `count` entries and
  * may be NULL.  Returns the first failing domain's status (message in
  * pearl_last_error); the others keep running to their own targets.  The
- * single-domain calls above act on domain 0. */
+ * single-domain calls above act on domain 0.  The domains' tiles together
+ * (256 x tile_rows cells each) are at most 2^32 / 512, one launch grid. */
 int pearl_batch_create(int count, const pearl_params* params, const double* mu_references, int precision,
                        int device, int cells_per_thread, pearl_ctx** out);
 int pearl_batch_size(const pearl_ctx* ctx, int* count);
