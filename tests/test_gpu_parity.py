@@ -143,6 +143,55 @@ def test_trajectory_tall_tiles(pkg, O, name, k):
     assert sc.to_list() == S.scalars()
 
 
+def mirrored_colony(g, rx, ry):
+    """The colony fixture repeated rx times in x and ry times in y, every
+    second copy flipped in y, so each seam joins equal rows and the state
+    stays a valid, smooth colony."""
+    def big(a):
+        col = np.concatenate([a if i % 2 == 0 else a[..., ::-1, :] for i in range(ry)], axis=-2)
+        return np.ascontiguousarray(np.tile(col, (1,) * (a.ndim - 1) + (rx,)))
+    return big(g["phi0"]), big(g["c0"]), big(g["mu0"])
+
+
+@pytest.mark.parametrize("phase_only", [0, 1])
+def test_many_tiles_bitwise(pkg, O, phase_only):
+    """2048x1152 cells in one-row tiles: 9,216 tiles, more than one round of
+    a finisher's partial fetch (8,192 partials per round for one or two
+    quantities, 4,096 for three or four, 2,048 for five), so every fold of
+    the blocked reductions runs several rounds.  Two whole steps (coupled:
+    propose, the PCG and the acceptance sums; phase-only: the five-quantity
+    acceptance fold) and the direct reductions are bitwise equal to the
+    oracle in the same blocked order."""
+    g = load_golden("colony_256x128")
+    rx, ry, k = 8, 9, 1
+    phi0, c0, mu0 = mirrored_colony(g, rx, ry)
+    pd = dict(g["pd"])
+    pd["nx"], pd["ny"], pd["phase_only"] = 256 * rx, 128 * ry, phase_only
+    mu_ref, end = float(g["sc0"][9]), float(g["end_time"])
+    say("many tiles", pd["nx"], pd["ny"], "k", k, "phase_only", phase_only)
+    with pkg.Solver(pd, mu_ref, cells_per_thread=k) as s:
+        s.set_state(phi0, c0, mu0, g["sc0"][:9])
+        assert s.layout()[1] == k and (pd["nx"] // 256) * pd["ny"] > 8192
+        m = s.total_carbon()
+        e = None if phase_only else s.reduced_energy()
+        infos = [s.advance(end) for _ in range(2)]
+        phi, c, mu, sc = s.get_state()
+    P = O.params_from_dict(pd, red_mode=O.RED_BLOCKED, red_k=k)
+    assert m == O.total_carbon(P, c0)
+    if not phase_only:
+        assert e == O.reduced_energy(P, phi0, c0, mu_ref)[0]
+    S = O.OracleState(phi0, c0, mu0, g["sc0"][:9])
+    for kk in range(2):
+        rc, oi = O.advance(P, S, mu_ref, end)
+        assert rc == 0
+        gi = infos[kk]
+        assert [gi.accepted_dt, gi.max_phase_change, gi.linear_residual, gi.closure, gi.energy_change,
+                gi.cg_iterations, gi.retries] == [oi.accepted_dt, oi.max_phase_change, oi.linear_residual,
+                                                  oi.closure, oi.energy_change, oi.cg_iterations, oi.retries], kk
+    assert np.array_equal(phi, S.phi) and np.array_equal(c, S.carbon) and np.array_equal(mu, S.mu)
+    assert sc.to_list() == S.scalars()
+
+
 @pytest.mark.parametrize("name", ["window_32x40", "window_alpha_32x40"])
 def test_moving_window(pkg, name):
     g = load_golden(name)
