@@ -581,7 +581,10 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
         int k = cells_per_thread;
         if (k <= 0) k = 4;
         if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return PEARL_ERR_ARGUMENT;
-        if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
+        if (hipSetDevice(device) != hipSuccess) {
+            (void)hipGetLastError();  // leave no error for the next call's launch check
+            return PEARL_ERR_DEVICE;
+        }
         c = new (std::nothrow) pearl_ctx();
         if (!c) return PEARL_ERR_ALLOC;
         c->device = device;
@@ -1618,7 +1621,13 @@ int pearl_stream_probe(int device, long long bytes, int reps, double* gbps) {
     if (!gbps || bytes <= 0 || bytes % 16 != 0 || bytes / 16 > (static_cast<long long>(1) << 31) || reps < 1)
         return PEARL_ERR_ARGUMENT;
     *gbps = 0;
-    if (hipSetDevice(device) != hipSuccess) return PEARL_ERR_DEVICE;
+    // The caller's current device is restored, and a failed call leaves no
+    // error behind for the next hipGetLastError of another entry point.
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return PEARL_ERR_DEVICE;
+    }
     const long long n = bytes / 16;
     probe_vec *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
@@ -1641,6 +1650,8 @@ int pearl_stream_probe(int device, long long bytes, int reps, double* gbps) {
     if (st) (void)hipStreamDestroy(st);
     if (b) (void)hipFree(b);
     if (a) (void)hipFree(a);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) (void)hipGetLastError();
     return e == hipSuccess ? PEARL_OK : (e == hipErrorOutOfMemory ? PEARL_ERR_ALLOC : PEARL_ERR_DEVICE);
 }
 
