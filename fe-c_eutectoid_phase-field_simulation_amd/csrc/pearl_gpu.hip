@@ -114,6 +114,9 @@ int alloc_fail(pearl_ctx* c) noexcept {
 }
 
 int hip_fail(pearl_ctx* c, hipError_t e, const char* where) {
+    // Reset the runtime's last error, so that a later call's launch check
+    // does not report this one again (a sticky device fault stays sticky).
+    (void)hipGetLastError();
     return fail(c, PEARL_ERR_DEVICE, std::string("HIP error in ") + where + ": " + hipGetErrorString(e));
 }
 
