@@ -629,6 +629,7 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
         const size_t total = plane * (3 * 2 + 2 + 2 + 11) + static_cast<size_t>(poff) + 64;
         auto bail = [&](int code) {
             pearl_destroy(c);
+            (void)hipGetLastError();  // the failed HIP call's error is reported here only
             return code;
         };
         if (hipMalloc(&c->mem, total * sizeof(double)) != hipSuccess) return bail(PEARL_ERR_ALLOC);
