@@ -178,32 +178,8 @@ def parity_with_reference_run(path, work, steps):
 
 
 def build_ensemble(pkg, count, seed, tmpdir):
-    """Seeded ensemble: widths from {256..2048}, heights from {128..1024},
-    undercooling U[5,40] K, 2..16 lamella pairs, cementite width by the lever
-    rule between the product contacts of the run temperature, colony a
-    quarter of the domain high."""
     import importlib
-    import numpy as np
-    inputs = importlib.import_module("pearl_amd.inputs")
-    config = importlib.import_module("pearl_amd.config")
-    initial = importlib.import_module("pearl_amd.initial")
-    doms = []
-    for spec in inputs.ensemble(count, seed):
-        nx, ny, uc, pairs = spec["nx"], spec["ny"], spec["undercooling"], spec["pairs"]
-        c_alpha = 0.00088690859310575857 + (-4.6282639461434345e-06) * (-uc)
-        f_theta = inputs.lever_rule_fraction(0.034458832049305582, c_alpha, 0.25)
-        strip = ny // 4
-        fills = inputs.lamella_fills(nx, pairs, strip, theta_fraction=f_theta, spacing_noise=0.05,
-                                     seed=seed * 1000 + spec["index"])
-        path = os.path.join(tmpdir, f"dom{spec['index']}.in")
-        inputs.write_input(path, nx, ny, 1e-6, 1e-6, fills,
-                           overrides={"undercooling": repr(float(uc)), "max_undercooling": "40"})
-        cfg = config.read_config(path)
-        phi = initial.diffuse_colony(nx, ny, cfg.dx, cfg.dy, cfg.eps, initial.plates_from_fills(fills), strip)
-        c = initial.carbon_at_mu(phi, cfg.thermo, cfg.initial_mu)
-        mu = np.full((ny, nx), cfg.initial_mu)
-        doms.append(dict(spec=spec, cfg=cfg, phi=phi, c=c, mu=mu))
-    return doms
+    return importlib.import_module("pearl_amd.inputs").ensemble_workload(count, seed, tmpdir)
 
 
 def run_mixed_accuracy(args, pkg):
@@ -588,14 +564,46 @@ def host_loop(s, cfg, args):
     return steps, shifts
 
 
-def mixed_beside(pkg, cfg, phi, c, mu, initial_carbon, args, have_torch, torch, nx, ny):
-    """The FP32-mixed form on the same workload: its throughput over the
-    same --steps accepted steps (same host loop), and its error against the
-    FP64 form over the first --mixed-lockstep steps in lockstep (the mixed
-    context takes each step at the FP64 context's accepted dt, so the
-    difference is precision alone): max |dmu| in J/mol and max |dphi| per
-    phase fraction."""
+def lockstep_errors(pkg, cfg, phi, c, mu, sc0, n_lock, every, k):
+    """FP64 and FP32-mixed contexts from the same state in lockstep: each
+    step the FP64 context advances and the mixed context takes a step of
+    exactly the accepted dt, so both follow one dt sequence and the
+    difference is precision alone.  Every `every` steps (and at the end):
+    max |dmu| in J/mol, max |dphi| per phase fraction, max |dc|."""
     import numpy as np
+    rows, broken = [], None
+    with pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=k) as a, \
+            pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=k, precision=pkg.MIXED) as b:
+        for z in (a, b):
+            z.set_state(phi, c, mu, sc0)
+        for step in range(1, n_lock + 1):
+            info = a.advance(1.0)
+            sc = b.scalars()
+            sc.dt = info.accepted_dt
+            b.set_scalars(sc)
+            r0 = b.counters()["rejected_attempts"]
+            b.advance(1.0)
+            if broken is None and (b.counters()["rejected_attempts"] != r0 or b.scalars().time != a.scalars().time):
+                broken = step
+            if step % every == 0 or step == n_lock:
+                pa, ca, ma, _ = a.get_state()
+                pb, cb, mb, _ = b.get_state()
+                rows.append({"step": step, "time": a.scalars().time,
+                             "max_abs_dmu_J_per_mol": float(np.max(np.abs(ma - mb))),
+                             "max_abs_dphi": [float(np.max(np.abs(pa[i] - pb[i]))) for i in range(3)],
+                             "max_abs_dc": float(np.max(np.abs(ca - cb)))})
+                log(f"  lockstep step {step}: max|dmu| {rows[-1]['max_abs_dmu_J_per_mol']:.3e} J/mol, "
+                    f"max|dphi| {rows[-1]['max_abs_dphi']}")
+    return {"lockstep_steps": n_lock, "lockstep_broken_at": broken, "every": rows,
+            "max_abs_dmu_J_per_mol": rows[-1]["max_abs_dmu_J_per_mol"], "max_abs_dphi": rows[-1]["max_abs_dphi"]}
+
+
+def mixed_beside(pkg, cfg, phi, c, mu, initial_carbon, args, have_torch, torch, nx, ny, tmp):
+    """The FP32-mixed form beside the FP64 one: its throughput on the same
+    workload over the same --steps accepted steps (same host loop), and its
+    error against the FP64 form over --mixed-lockstep steps in lockstep
+    (independent of --steps), on this workload and on the 8192x4096 one
+    (64 pairs, 20 K undercooling, same seed), reported every 1,000 steps."""
     sc0 = [0.0, cfg.dt, 0.0, initial_carbon, 0.0, 0.0, 0, 0, 0]
     out = {}
     with pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread,
@@ -615,29 +623,24 @@ def mixed_beside(pkg, cfg, phi, c, mu, initial_carbon, args, have_torch, torch, 
         out.update(value=round(nx * ny * args.steps / wall / 1e6, 3), ms_per_step=round(wall * 1e3 / args.steps, 3),
                    steps=args.steps, attempts=c1["attempts"] - c0["attempts"],
                    cg_iterations=c1["cg_iterations"] - c0["cg_iterations"])
-    n_lock = min(args.mixed_lockstep, args.steps)
-    with pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread) as a, \
-            pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread,
-                       precision=pkg.MIXED) as b:
-        for z in (a, b):
-            z.set_state(phi, c, mu, sc0)
-        broken = None
-        for k in range(1, n_lock + 1):
-            info = a.advance(1.0)
-            sc = b.scalars()
-            sc.dt = info.accepted_dt
-            b.set_scalars(sc)
-            r0 = b.counters()["rejected_attempts"]
-            b.advance(1.0)
-            if broken is None and (b.counters()["rejected_attempts"] != r0 or b.scalars().time != a.scalars().time):
-                broken = k
-        pa, _, ma, _ = a.get_state()
-        pb, _, mb, _ = b.get_state()
-        out.update(lockstep_steps=n_lock, lockstep_broken_at=broken,
-                   max_abs_dmu_J_per_mol=float(np.max(np.abs(ma - mb))),
-                   max_abs_dphi=[float(np.max(np.abs(pa[i] - pb[i]))) for i in range(3)])
-    log(f"mixed form: {out['value']:.1f} Mcell-steps/s; after {n_lock} lockstep steps max|dmu| "
-        f"{out['max_abs_dmu_J_per_mol']:.3e} J/mol, max|dphi| {out['max_abs_dphi']}")
+    n_lock = args.mixed_lockstep
+    if n_lock > 0:
+        log(f"mixed form: lockstep against FP64 on {nx}x{ny}, {n_lock} steps")
+        out["lockstep"] = lockstep_errors(pkg, cfg, phi, c, mu, sc0, n_lock, 1000, args.cells_per_thread)
+        out["lockstep"]["workload"] = f"{nx}x{ny}, {args.pairs} pairs, {args.undercooling} K (this workload)"
+        nx2, ny2, pairs2, uc2 = 8192, 4096, 64, 20.0
+        log(f"mixed form: lockstep against FP64 on {nx2}x{ny2}, {pairs2} pairs, {uc2} K, {n_lock} steps")
+        _, cfg2, phi2, c2, mu2 = build_workload(pkg, nx2, ny2, pairs2, ny2 // 8, args.seed, uc2, tmp)
+        with pkg.Solver(cfg2.solver_params(), cfg2.initial_mu) as z:
+            z.set_state(phi2, c2, mu2, [0.0, cfg2.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
+            m2 = z.total_carbon()
+        lk = lockstep_errors(pkg, cfg2, phi2, c2, mu2, [0.0, cfg2.dt, 0.0, m2, 0.0, 0.0, 0, 0, 0], n_lock, 1000,
+                             args.cells_per_thread)
+        lk["workload"] = f"{nx2}x{ny2}, {pairs2} pairs +/-5% (seed {args.seed}), {uc2} K"
+        out["lockstep_8192x4096_20K"] = lk
+        out["max_abs_dmu_J_per_mol"] = max(out["lockstep"]["max_abs_dmu_J_per_mol"], lk["max_abs_dmu_J_per_mol"])
+        del phi2, c2, mu2
+    log(f"mixed form: {out['value']:.1f} Mcell-steps/s")
     return out
 
 
@@ -657,7 +660,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="production: skip the FP32-mixed run beside the FP64 one")
     ap.add_argument("--mixed-lockstep", type=int, default=1000,
-                    help="production: lockstep steps for the mixed form's stated error")
+                    help="production: lockstep steps for the mixed form's stated error (on this workload and on "
+                         "the 8192x4096, 64-pair, 20 K one; independent of --steps; 0 skips it)")
     ap.add_argument("--kernel-steps", type=int, default=500,
                     help="accepted steps of the per-kernel event region after the timed one")
     ap.add_argument("--no-profile", action="store_true",
@@ -851,7 +855,8 @@ def main():
     s.close()
     if args.precision == "fp64" and not args.no_mixed:
         # the metric's other half: the FP32-mixed form at its stated mu error
-        mx = mixed_beside(pkg, cfg, phi, c, mu, initial_carbon, args, have_torch, torch if have_torch else None, nx, ny)
+        mx = mixed_beside(pkg, cfg, phi, c, mu, initial_carbon, args, have_torch, torch if have_torch else None, nx, ny,
+                          tmp)
         mx["over_fp64"] = round(mx["value"] / out["value"], 3)
         out["mixed"] = mx
     if not args.no_cpu_baseline:

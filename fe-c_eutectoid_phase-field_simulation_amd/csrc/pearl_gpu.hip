@@ -461,6 +461,12 @@ bool params_ok(const pearl_params* p) {
     for (int i = 0; i < 3; ++i)
         if (!(p->A[i] > 0) || !std::isfinite(p->A[i]) || !(p->tau[i] > 0)) return false;
     if (!(p->eps > 0) || !(p->sigma > 0) || !(p->Vm > 0) || p->triple < 0) return false;
+    // the kernels' constant divisors (dx, dy, their squares, Vm, 2A) lie in
+    // [2^-150, 2^150], the range the fast constant-divisor quotient assumes
+    auto in = [](double v, double lo, double hi) { return v >= lo && v <= hi; };
+    if (!in(p->dx, 0x1p-75, 0x1p75) || !in(p->dy, 0x1p-75, 0x1p75) || !in(p->Vm, 0x1p-150, 0x1p150)) return false;
+    for (int i = 0; i < 3; ++i)
+        if (!in(2.0 * p->A[i], 0x1p-150, 0x1p150)) return false;
     for (int i = 0; i < 3; ++i)
         if (!(p->pair_sigma[i] >= 0) || !std::isfinite(p->pair_sigma[i])) return false;
     return true;
@@ -891,6 +897,10 @@ int pearl_maybe_shift(pearl_ctx* c, int moving_window, double trigger, double ta
         }
         if (rows <= 0 || rows >= P.ny || c_gamma < 0 || c_gamma > 1) return fail(c, PEARL_ERR_INVALID_SHIFT, "invalid shift");
         const double added = rows * P.nx * c_gamma * P.dx * P.dy;
+        // the flags k_shift raises start clear: an earlier failed shift
+        // leaves them set
+        hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(64), 0, c->stream, c->B, 0,
+                           static_cast<int>(ERRF_MIXTURE | ERRF_NONFINITE_CARBON));
         PEARL_LAUNCH(c, k_shift, dim3(P.nb), c->B, 0, rows, c_gamma);
         st = read_scalars(c);
         if (st) return st;

@@ -130,16 +130,24 @@ __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b
 // (1/2 + 2^-51) ulp of a/d, so q1 is a faithful rounding.  For a faithful q
 // and a reciprocal with relative error below 2^-53, Markstein's theorem
 // (P. Markstein, IBM J. Res. Dev. 34(1), 1990) gives RN(q + (a - d q) y) =
-// RN(a/d): the second correction returns the IEEE quotient.  Valid while
-// a, the quotient and the remainders are normal.  As d > 0 the quotient
-// has the sign of a; copysign restores it for a zero numerator, where the
-// corrections may turn -0 into +0.  One multiply, four FMAs and a bitfield
-// insert instead of the scaled reciprocal iteration of a general division.
+// RN(a/d): the second correction returns the IEEE quotient.  The argument
+// needs the remainders a - q d exact (|a| >= 2^-969) and the quotient normal
+// and finite.  The host admits only divisors in [2^-150, 2^150], so both
+// hold when a = 0 or 2^-818 <= |a| < 2^750: frexp's exponent of a in
+// [-817, 750] (0 for a = 0).  Any other numerator -- subnormal, tiny,
+// huge -- takes the IEEE division itself.  An infinite or NaN numerator
+// (frexp exponent 0) stays on the fast path, and div_fixup turns its result
+// into the IEEE one (inf with the sign of a, NaN); for every other a it
+// returns |q2| with the sign of a (d > 0), which also gives a zero
+// numerator its sign back.  In range: one multiply, four FMAs and the
+// fix-up instead of the scaled reciprocal iteration of a general division.
 __device__ __forceinline__ double div_const(double a, double d, double y) {
+    const int e = __builtin_amdgcn_frexp_exp(a);
+    if (__builtin_expect(static_cast<unsigned>(e + 817) > 1567u, 0)) return a / d;
     const double q0 = a * y;
     const double q1 = __builtin_fma(__builtin_fma(-q0, d, a), y, q0);
     const double q2 = __builtin_fma(__builtin_fma(-q1, d, a), y, q1);
-    return __builtin_copysign(q2, a);
+    return __builtin_amdgcn_div_fixup(q2, d, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -1847,7 +1855,11 @@ __global__ __launch_bounds__(TPB) void k_iface_energy_fin(Bufs B, int d) {
 }
 
 // Clears error bits (a kernel, not a memset, so it is safe in graphs).
-__global__ void k_clear_err(Bufs B, int d) { B.S[d].err = 0; }
+// Clears the ERRF_* bits `bits` of domain d (one thread), so that a call
+// reports only what its own kernels flag.
+__global__ void k_clear_err(Bufs B, int d, int bits) {
+    if (threadIdx.x == 0) B.S[d].err &= ~bits;
+}
 
 // ---------------------------------------------------------------------------
 // Initialization (initialize :1761-1837): the sharp labels of the FILL

@@ -151,3 +151,36 @@ def colony_workload(nx: int, ny: int, pairs: int, strip_rows: int, seed: int, un
     c = initial.carbon_at_mu(phi, cfg.thermo, cfg.initial_mu)
     mu = np.full((ny, nx), cfg.initial_mu)
     return path, cfg, phi, c, mu
+
+
+def ensemble_workload(count: int, seed: int, directory: str):
+    """The seeded batched-ensemble workload: `count` colonies from
+    ensemble(count, seed) (widths 256..2048, heights 128..1024,
+    undercooling U[5, 40] K, 2..16 lamella pairs), each with the cementite
+    width of the lever rule between the product contacts of its run
+    temperature, spacing +/-5 %, colony a quarter of the domain high.
+    Returns one dict per domain: spec, cfg, phi[3,ny,nx], c, mu (the diffuse
+    initial state at the input's initial mu)."""
+    import os
+
+    import numpy as np
+
+    from . import config, initial
+    doms = []
+    for spec in ensemble(count, seed):
+        nx, ny, uc, pairs = spec["nx"], spec["ny"], spec["undercooling"], spec["pairs"]
+        # ferrite contact at the run temperature (the default input's linear law)
+        c_alpha = 0.00088690859310575857 + (-4.6282639461434345e-06) * (-uc)
+        f_theta = lever_rule_fraction(0.034458832049305582, c_alpha, 0.25)
+        strip = ny // 4
+        fills = lamella_fills(nx, pairs, strip, theta_fraction=f_theta, spacing_noise=0.05,
+                              seed=seed * 1000 + spec["index"])
+        path = os.path.join(directory, f"dom{spec['index']}.in")
+        write_input(path, nx, ny, 1e-6, 1e-6, fills,
+                    overrides={"undercooling": repr(float(uc)), "max_undercooling": "40"})
+        cfg = config.read_config(path)
+        phi = initial.diffuse_colony(nx, ny, cfg.dx, cfg.dy, cfg.eps, initial.plates_from_fills(fills), strip)
+        c = initial.carbon_at_mu(phi, cfg.thermo, cfg.initial_mu)
+        mu = np.full((ny, nx), cfg.initial_mu)
+        doms.append(dict(spec=spec, cfg=cfg, phi=phi, c=c, mu=mu))
+    return doms

@@ -63,11 +63,12 @@ extern "C" {
 typedef struct {
     int nx, ny;                        /* global cells; x periodic, y zero flux; nx >= 3, ny >= 2,
                                           3*nx*ny < 2^31 - 2^20 */
-    double dx, dy;                     /* m */
-    double A[3], B[3], C[3], D[3];     /* f_p(c)=A c^2+B c+C [J/mol]; D_p [m^2/s] */
+    double dx, dy;                     /* m, each in [2^-75, 2^75] */
+    double A[3], B[3], C[3], D[3];     /* f_p(c)=A c^2+B c+C [J/mol], 2A in [2^-150, 2^150];
+                                          D_p [m^2/s] */
     double eps, sigma, triple;         /* interface parameters */
     double tau[3];                     /* tau_at, tau_ag, tau_tg [J s/m^4] */
-    double Vm;                         /* m^3/mol */
+    double Vm;                         /* m^3/mol, in [2^-150, 2^150] */
     double dt_max, dt_min, max_phase_change;
     double cg_rtol, cg_atol, closure_tol;
     double energy_rtol, energy_atol;

@@ -8,6 +8,12 @@ TEST INFRASTRUCTURE ONLY (tests/ and the CPU-baseline leg of bench.py).
   python oracle/ref_tool.py read-checkpoint INPUT.in FRAME.h5 OUT.npz
       load a checkpoint with the reference reader (signature, closure and
       ledger checks included) and save phi, carbon, mu, scalars
+  python oracle/ref_tool.py sharp-labels INPUT.in OUT.npz
+      the phases of the reference's initialize at its init_sharp observer
+      call (the FILL labels before smoothing)
+  python oracle/ref_tool.py initialize INPUT.in OUT.npz
+      the reference's whole initialize (labels, smoothing, carbon set-up):
+      phi, carbon, mu, scalars
   python oracle/ref_tool.py baseline INPUT.in STATE.npz SECONDS OUT.json
       time the reference advance() from the saved state, serially
   python oracle/ref_tool.py baseline-mpi INPUT.in STATE.npz NPROCS STEPS OUT.json
@@ -76,6 +82,21 @@ def read_checkpoint(inp, h5, tmpdir):
     return z["phi"], z["carbon"], z["mu"], z["scalars"]
 
 
+def sharp_labels(inp, tmpdir):
+    """phi[3, ny, nx] of the reference's FILL labels for input file `inp`."""
+    out = os.path.join(tmpdir, os.path.basename(inp) + ".labels.npz")
+    call("sharp-labels", inp, out)
+    return np.load(out)["phi"]
+
+
+def initialize(inp, tmpdir):
+    """phi, carbon, mu, scalars of the reference's initialize for `inp`."""
+    out = os.path.join(tmpdir, os.path.basename(inp) + ".init.npz")
+    call("initialize", inp, out)
+    z = np.load(out)
+    return z["phi"], z["carbon"], z["mu"], z["scalars"]
+
+
 def main(argv):
     sys.path.insert(0, HERE)
     import oracle as O
@@ -84,6 +105,17 @@ def main(argv):
         inp, h5, out = argv[1:4]
         R = O.Ref(inp)
         R.read_checkpoint(h5)
+        phi, c, mu, sc = R.get_state()
+        np.savez(out, phi=phi, carbon=c, mu=mu, scalars=sc)
+        return 0
+    if cmd == "sharp-labels":
+        inp, out = argv[1:3]
+        np.savez(out, phi=O.Ref(inp).sharp_labels())
+        return 0
+    if cmd == "initialize":
+        inp, out = argv[1:3]
+        R = O.Ref(inp)
+        R.initialize()
         phi, c, mu, sc = R.get_state()
         np.savez(out, phi=phi, carbon=c, mu=mu, scalars=sc)
         return 0

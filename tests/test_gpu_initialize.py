@@ -32,6 +32,7 @@ import pytest
 
 import fill_recipes as FR
 import oracle
+import ref_tool
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not oracle.ref_available(), reason="reference build (oracle/_ref) missing")]
@@ -64,7 +65,7 @@ def test_fill_labels_equal_reference(pkg, tmp_path, nx, ny, k, whole_map):
         say("fill", nx, ny, whole_map, "seed", seed, "k", k)
         path = recipe_input(tmp_path, nx, ny, seed, whole_map)
         cfg = mod("config").read_config(path)
-        want = oracle.Ref(path).sharp_labels()
+        want = ref_tool.sharp_labels(path, str(tmp_path))
         ops, n = pkg.fill_ops(cfg)
         c0 = np.linspace(0.01, 0.05, nx * ny)
         mu0 = np.full(nx * ny, cfg.initial_mu)
@@ -87,7 +88,7 @@ def test_sharp_smooth_labels_equal_reference(pkg, tmp_path, nx, ny, theta_column
     with pkg.Solver(cfg.solver_params(), cfg.initial_mu) as s:
         s.fill_phases(ops, n)
         phi = s.get_state()[0]
-    assert np.array_equal(phi, oracle.Ref(path).sharp_labels())
+    assert np.array_equal(phi, ref_tool.sharp_labels(path, str(tmp_path)))
 
 
 @pytest.mark.parametrize("nx,ny,pairs,strip", [(4096, 2048, 32, 256), (8192, 4096, 64, 512)])
@@ -131,9 +132,7 @@ def test_composition_equals_reference(pkg, tmp_path, mode):
     say("composition", mode)
     path = composition_input(tmp_path, mode)
     cfg = mod("config").read_config(path)
-    R = oracle.Ref(path)
-    R.initialize()
-    phi_r, c_r, mu_r, sc_r = R.get_state()
+    phi_r, c_r, mu_r, sc_r = ref_tool.initialize(path, str(tmp_path))
     n = cfg.nx * cfg.ny
     mean = cfg.composition_mode == "mean"
     with pkg.Solver(cfg.solver_params(), cfg.initial_mu) as s:
@@ -161,9 +160,7 @@ def test_device_initialize_equals_reference(pkg, tmp_path, seed, whole_map):
     say("device initialize", seed, whole_map)
     path = recipe_input(tmp_path, 300, 170, seed, whole_map, smooth_steps=40, smooth_min_steps=40)
     cfg = mod("config").read_config(path)
-    R = oracle.Ref(path)
-    R.initialize()
-    phi_r, c_r, mu_r, sc_r = R.get_state()
+    phi_r, c_r, mu_r, sc_r = ref_tool.initialize(path, str(tmp_path))
     ops, n = pkg.fill_ops(cfg)
     with pkg.Solver(cfg.solver_params(), cfg.initial_mu) as s:
         s.fill_phases(ops, n)

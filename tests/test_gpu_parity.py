@@ -215,6 +215,35 @@ def test_moving_window(pkg, name):
     assert abs(got[5] - ref[5]) <= 1e-12 * abs(ref[5])
 
 
+def test_shift_after_failed_shift(pkg):
+    """A shift that fails inside the row move (a non-finite carbon value
+    moved down: "Nonfinite total carbon") leaves no flag behind: a later
+    shift that removes that cell instead succeeds and equals the same shift
+    on a fresh context."""
+    g = load_golden("window_alpha_32x40")
+    trigger, target, c_gamma = [float(v) for v in g["window"]]
+    dy = float(g["pd"]["dy"])
+    nx = int(g["pd"]["nx"])
+    c1 = np.array(g["c1"], dtype=np.float64).reshape(-1)
+    c1[20 * nx + 5] = np.nan  # row 20: moved by the 18-row shift, removed by a 22-row one
+    say("shift after a failed shift")
+    with pkg.Solver(g["pd"], float(g["sc0"][9])) as s:
+        s.set_state(g["phi1"], c1, g["mu1"], g["sc1"][:9])
+        with pytest.raises(pkg.PearlError) as ei:
+            s.maybe_shift(True, trigger, target, c_gamma)
+        assert str(ei.value) == "Nonfinite total carbon"
+        shifted, _ = s.maybe_shift(True, trigger, target - 4 * dy, c_gamma)
+        assert shifted
+        got = s.get_state()
+    with pkg.Solver(g["pd"], float(g["sc0"][9])) as s:
+        s.set_state(g["phi1"], c1, g["mu1"], g["sc1"][:9])
+        assert s.maybe_shift(True, trigger, target - 4 * dy, c_gamma)[0]
+        want = s.get_state()
+    for a, b in zip(got[:3], want[:3]):
+        assert np.all(np.isfinite(a)) and np.array_equal(a, b)
+    assert got[3].shifts == want[3].shifts == 1
+
+
 def test_diagnostics(pkg, O):
     g = load_golden("small_48x40")
     say("diagnostics")

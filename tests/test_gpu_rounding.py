@@ -2,7 +2,9 @@
 be correctly rounded (bitwise equal to the host's IEEE results), and the
 solver's constant-divisor quotient (div_const) must equal the IEEE quotient:
 random numerators over a wide exponent range, numerators whose quotients
-lie next to rounding midpoints, signed zeros, for the solver's divisors
+lie next to rounding midpoints, signed zeros, subnormal numerators,
+numerators and quotients next to the normal boundary and next to overflow,
+infinities and NaN, for the solver's divisors
 (dx, dy, dx^2, dy^2, Vm, 2A, 3) and for divisors with extreme
 significands."""
 import ctypes as C
@@ -63,16 +65,34 @@ def test_division_correctly_rounded():
 def test_constant_divisor_quotient(d):
     rng = np.random.default_rng(9)
     n = 1 << 21
-    a = np.concatenate([random_doubles(rng, n, -250, 250) * rng.choice([-1, 1], n),
-                        rng.normal(size=n), rng.normal(size=n) * 1e-20,
-                        # quotients next to rounding midpoints: a = (m + 1/2 ulp) * d rounded
-                        (random_doubles(rng, n, -40, 40) * d),
-                        np.nextafter(random_doubles(rng, n, -40, 40) * d, np.inf),
-                        np.array([0.0, -0.0, 1.0, -1.0, d, -d])])
+    m = 1 << 16
+    sign = rng.choice([-1, 1], m)
+    tiny = np.finfo(np.float64).smallest_subnormal
+    with np.errstate(over="ignore"):
+        a = np.concatenate([random_doubles(rng, n, -250, 250) * rng.choice([-1, 1], n),
+                            rng.normal(size=n), rng.normal(size=n) * 1e-20,
+                            # quotients next to rounding midpoints: a = (m + 1/2 ulp) * d rounded
+                            (random_doubles(rng, n, -40, 40) * d),
+                            np.nextafter(random_doubles(rng, n, -40, 40) * d, np.inf),
+                            # subnormal numerators, numerators and quotients next to the
+                            # normal boundary and next to overflow
+                            sign * rng.integers(1, 1 << 52, m) * tiny,
+                            sign * random_doubles(rng, m, -1075, -940),
+                            sign * random_doubles(rng, m, -1022, -1020),
+                            sign * random_doubles(rng, m, 990, 1023),
+                            sign * random_doubles(rng, m, -960, -900) * d,
+                            sign * np.nextafter(np.ldexp(1.0, -960) * d, 0.0) * rng.random(m),
+                            sign * np.minimum(random_doubles(rng, m, 1015, 1024) * d, np.finfo(np.float64).max),
+                            np.array([0.0, -0.0, 1.0, -1.0, d, -d, tiny, -tiny, np.inf, -np.inf, np.nan,
+                                      np.finfo(np.float64).max, -np.finfo(np.float64).max,
+                                      np.finfo(np.float64).tiny, np.ldexp(1.0, -960), np.ldexp(1.0, 1000)])])
     a = np.ascontiguousarray(a)
     q = np.empty_like(a)
     assert lib().probe_div_const(p(a), d, 1.0 / d, p(q), a.size) == 0
-    ref = a / d
-    bad = np.flatnonzero(q != ref)
+    with np.errstate(over="ignore", under="ignore", invalid="ignore"):
+        ref = a / d
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(q), nan)
+    bad = np.flatnonzero((q != ref) & ~nan)
     assert bad.size == 0, (bad.size, a[bad[:5]])
-    assert np.array_equal(np.signbit(q), np.signbit(ref))
+    assert np.array_equal(np.signbit(q[~nan]), np.signbit(ref[~nan]))
