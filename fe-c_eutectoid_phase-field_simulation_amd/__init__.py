@@ -166,7 +166,8 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_get_layout", "pearl_profile", "pearl_kernel_stats", "pearl_kernel_name", "pearl_counters",
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
            "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance",
-           "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition", "pearl_stream_probe"]
+           "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition", "pearl_stream_probe",
+           "pearl_diagnostics_noalloc"]
 
 
 def lib():
@@ -189,6 +190,8 @@ def lib():
                                 C.POINTER(C.c_int), C.POINTER(StepInfo)]
         L.pearl_maybe_shift.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), _dp]
         L.pearl_diagnostics.argtypes = [vp, C.POINTER(Diag)]
+        L.pearl_diagnostics_noalloc.argtypes = [C.POINTER(Params), C.c_double, C.c_int, vp, vp, vp, vp,
+                                                C.POINTER(Diag), C.c_char_p, C.c_int]
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
         L.pearl_fill_phases.argtypes = [vp, C.POINTER(Fill), C.c_int]
         L.pearl_stream_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
@@ -237,6 +240,21 @@ def stream_probe(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> floa
     if rc != OK:
         raise PearlError(rc, "pearl_stream_probe failed")
     return g.value
+
+
+def diagnostics_noalloc(params, mu_reference: float, phi_ptr: int, carbon_ptr: int, mu_ptr: int,
+                        cells_per_thread: int = 0, stream: int = 0) -> dict:
+    """pearl_diagnostics_noalloc: the snapshot reductions of a state in the
+    caller's device memory (device addresses of phi[3,ny,nx], carbon, mu,
+    float64, contiguous) on the current device; allocates no device memory."""
+    P = params if isinstance(params, Params) else params_from_dict(params)
+    d = Diag()
+    msg = C.create_string_buffer(256)
+    rc = lib().pearl_diagnostics_noalloc(C.byref(P), mu_reference, cells_per_thread, phi_ptr, carbon_ptr, mu_ptr,
+                                         stream or None, C.byref(d), msg, 256)
+    if rc != OK:
+        raise PearlError(rc, msg.value.decode())
+    return {k: getattr(d, k) for k, _ in Diag._fields_}
 
 
 def params_from_dict(d: dict) -> Params:

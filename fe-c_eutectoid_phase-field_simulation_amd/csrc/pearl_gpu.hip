@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <cmath>
 #include <cstddef>
@@ -1667,6 +1668,130 @@ int pearl_stream_probe(int device, long long bytes, int reps, double* gbps) {
     (void)hipSetDevice(prev);
     if (e != hipSuccess) (void)hipGetLastError();
     return e == hipSuccess ? PEARL_OK : (e == hipErrorOutOfMemory ? PEARL_ERR_ALLOC : PEARL_ERR_DEVICE);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// pearl_diagnostics_noalloc: the snapshot reductions of a state held in the
+// caller's device memory, for callers that cannot allocate device memory.
+// Its device memory beyond the inputs -- the parameter block, the device
+// scalars that receive the totals, and the tile partials of the blocked
+// reduction -- is one file-scope array per device, `scratch`, held until the
+// process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).  Every call on a device shares
+// that device's array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr long long kNoAllocTiles = 65536;  // tiles of 256 x k cells
+constexpr size_t kNoAllocKP = 0, kNoAllocS = 2048, kNoAllocPart = 4096;
+constexpr size_t kNoAllocBytes = kNoAllocPart + 8 * static_cast<size_t>(kNoAllocTiles) * sizeof(double);
+static_assert(sizeof(KP) <= kNoAllocS - kNoAllocKP && sizeof(DevScal) <= kNoAllocPart - kNoAllocS,
+              "parameter block or device scalars outgrew their scratch slots");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_fail(char* msg, int msg_len, int code, const char* text) {
+    if (msg && msg_len > 0) std::snprintf(msg, static_cast<size_t>(msg_len), "%s", text);
+    return code;
+}
+
+// The current device's scratch address, from hipGetSymbolAddress on its
+// first use on that device.
+hipError_t scratch_of_current_device(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int pearl_diagnostics_noalloc(const pearl_params* params, double mu_reference, int cells_per_thread,
+                              const double* phi, const double* carbon, const double* mu, void* stream,
+                              pearl_diag* out, char* msg, int msg_len) {
+    if (msg && msg_len > 0) msg[0] = 0;
+    if (!params || !phi || !carbon || !mu || !out) return noalloc_fail(msg, msg_len, PEARL_ERR_ARGUMENT, "null argument");
+    if (!params_ok(params)) return noalloc_fail(msg, msg_len, PEARL_ERR_ARGUMENT, "invalid parameters");
+    int k = cells_per_thread <= 0 ? 4 : cells_per_thread;
+    if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16)
+        return noalloc_fail(msg, msg_len, PEARL_ERR_ARGUMENT, "cells_per_thread must be 1, 2, 4, 8 or 16");
+    KP P = make_kp(params, mu_reference, k);
+    if (P.nb > kNoAllocTiles) {
+        char text[160];
+        std::snprintf(text, sizeof(text), "domain of %d tiles exceeds the no-allocation scratch (%lld tiles of 256 x %d cells)",
+                      P.nb, kNoAllocTiles, k);
+        return noalloc_fail(msg, msg_len, PEARL_ERR_ARGUMENT, text);
+    }
+    unsigned char* base = nullptr;
+    hipError_t e = scratch_of_current_device(&base);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return noalloc_fail(msg, msg_len, PEARL_ERR_DEVICE, hipGetErrorString(e));
+    }
+    // parameter block and cleared device scalars (cur = 0: the state below
+    // is "current"); the host copies live until the stream is synchronised
+    P.off = 0;
+    P.poff = 0;
+    P.boff = 0;
+    DevScal hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.phase = PH_IDLE;
+    Bufs B{};
+    for (int b = 0; b < 2; ++b) {
+        B.phi[b] = const_cast<double*>(phi);
+        B.c[b] = const_cast<double*>(carbon);
+        B.mu[b] = const_cast<double*>(mu);
+    }
+    B.kp = reinterpret_cast<const KP*>(base + kNoAllocKP);
+    B.S = reinterpret_cast<DevScal*>(base + kNoAllocS);
+    B.part = reinterpret_cast<double*>(base + kNoAllocPart);
+    B.ndom = 1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    e = hipMemcpyAsync(base + kNoAllocKP, &P, sizeof(KP), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + kNoAllocS, &hs, sizeof(DevScal), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const dim3 g(static_cast<unsigned>(P.nb)), tb(TPB);
+        switch (k) {
+            case 1: hipLaunchKernelGGL(k_diagnostics<1>, g, tb, 0, st, B, 0); break;
+            case 2: hipLaunchKernelGGL(k_diagnostics<2>, g, tb, 0, st, B, 0); break;
+            case 4: hipLaunchKernelGGL(k_diagnostics<4>, g, tb, 0, st, B, 0); break;
+            case 8: hipLaunchKernelGGL(k_diagnostics<8>, g, tb, 0, st, B, 0); break;
+            default: hipLaunchKernelGGL(k_diagnostics<16>, g, tb, 0, st, B, 0); break;
+        }
+        hipLaunchKernelGGL(k_diagnostics_fin, dim3(1), tb, 0, st, B, 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&hs, base + kNoAllocS, sizeof(DevScal), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return noalloc_fail(msg, msg_len, PEARL_ERR_DEVICE, hipGetErrorString(e));
+    }
+    if (!params->phase_only && (hs.err & ERRF_MIXTURE_STATE))
+        return noalloc_fail(msg, msg_len, PEARL_ERR_MIXTURE, "Invalid mixture coefficients");
+    out->c_min = hs.dmin_c;
+    out->c_max = hs.dmax_c;
+    out->mu_min = hs.dmin_mu;
+    out->mu_max = hs.dmax_mu;
+    out->phi_sum_error = hs.dsumerr;
+    out->mass = hs.dmass * P.dx * P.dy;
+    out->energy = hs.denergy * P.dx * P.dy;
+    return PEARL_OK;
 }
 
 int pearl_counters(const pearl_ctx* c, long long* attempts, long long* rejected, long long* cg) {

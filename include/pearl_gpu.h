@@ -189,6 +189,24 @@ int pearl_maybe_shift(pearl_ctx* ctx, int moving_window, double shift_trigger, d
 /* Snapshot reductions (:2267-2282). */
 int pearl_diagnostics(pearl_ctx* ctx, pearl_diag* out);
 
+/* Snapshot reductions without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same reductions and results as pearl_diagnostics, on a
+ * state held in the caller's device memory: phi (three [ny][nx] planes,
+ * alpha, theta, gamma), carbon and mu ([ny][nx]), on the current HIP device.
+ * No context: the parameter block, the device scalars and the tile
+ * partials live in one static device array per device (about 4.2 MB, held
+ * until the process ends), so the domain may have at most 65,536 tiles of
+ * 256 x cells_per_thread cells (0 = 4; 67 M cells at 4); larger domains are
+ * rejected with PEARL_ERR_ARGUMENT.  The call allocates nothing, runs on
+ * `stream` (a hipStream_t; NULL = the default stream) and synchronises it.
+ * Calls that overlap on one device share that array and race.  Errors carry
+ * their message in msg (msg_len bytes, may be NULL). */
+int pearl_diagnostics_noalloc(const pearl_params* params, double mu_reference, int cells_per_thread,
+                              const double* phi, const double* carbon, const double* mu, void* stream,
+                              pearl_diag* out, char* msg, int msg_len);
+
 /* Single reference functions on the current state, for parity tests.
  *   pearl_propose_phi        propose_phi :961 (phi_out: 3 planes, may be NULL)
  *   pearl_implicit_diffusion implicit_diffusion :1010 with a given trial phi
