@@ -206,9 +206,9 @@ def run_mixed_accuracy(args, pkg):
     log(f"mixed accuracy ({args.acc_mode}): {nx}x{ny}, {pairs} pairs, {uc} K, {args.acc_steps} FP64 steps, "
         f"compared every {args.acc_every}")
     a = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=pkg.FP64)
-    if args.acc_other == "mixed":
+    if args.acc_other in ("mixed", "stencil32"):
         b = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread,
-                       precision=pkg.MIXED)
+                       precision=pkg.MIXED if args.acc_other == "mixed" else pkg.FP32_STENCIL)
     else:
         b = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=8, precision=pkg.FP64)
     for s in (a, b):
@@ -681,14 +681,14 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=2000, help="adjoint: fixed steps")
     ap.add_argument("--adj-every", type=int, default=100, help="adjoint: checkpoint interval")
     ap.add_argument("--adj-no-reference", action="store_true", help="adjoint: skip the torch autograd reference")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+    ap.add_argument("--precision", choices=["fp64", "mixed", "stencil32"], default="fp64",
                     help="mixed: Jacobi diagonal and PCG vectors p, z stored in FP32 (state, operator "
                          "coefficients, residuals, dots, mass and acceptance FP64)")
     ap.add_argument("--acc-steps", type=int, default=3000, help="mixed-accuracy: FP64 steps in total")
     ap.add_argument("--acc-every", type=int, default=1000, help="mixed-accuracy: steps between comparisons")
     ap.add_argument("--acc-mode", choices=["lockstep", "free"], default="lockstep",
                     help="mixed-accuracy: one dt sequence, or each form on its own dt compared at matched times")
-    ap.add_argument("--acc-other", choices=["mixed", "fp64-k8"], default="mixed",
+    ap.add_argument("--acc-other", choices=["mixed", "fp64-k8", "stencil32"], default="mixed",
                     help="mixed-accuracy: the form compared with FP64 at the default tile height; fp64-k8 is the "
                          "FP64 form with another summation order (8-row tiles), the noise floor of the comparison")
     ap.add_argument("--domains", type=int, default=64)
@@ -739,7 +739,7 @@ def main():
     path, cfg, phi, c, mu = build_workload(pkg, nx, ny, args.pairs, strip_rows, args.seed, args.undercooling, tmp,
                                            overrides=window)
     log(f"workload {nx}x{ny}: {args.pairs} pairs, colony {strip_rows} rows, {args.undercooling} K")
-    precision = pkg.MIXED if args.precision == "mixed" else pkg.FP64
+    precision = {"fp64": pkg.FP64, "mixed": pkg.MIXED, "stencil32": pkg.FP32_STENCIL}[args.precision]
     s = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=precision)
     s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
     initial_carbon = s.total_carbon()

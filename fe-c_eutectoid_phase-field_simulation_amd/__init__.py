@@ -32,6 +32,7 @@ WHY = ["phase increment", "linear/closure", "composition/phase bounds", "mass co
        "free-energy increase"]
 FP64 = 0
 MIXED = 1  # Jacobi diagonal, p, z in FP32; state, operator, residuals, dots and acceptance FP64
+FP32_STENCIL = 2  # MIXED plus FP32 arithmetic in the phase proposal's stencil (force, mean, step scale)
 
 KERNELS = ["propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
            "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"]

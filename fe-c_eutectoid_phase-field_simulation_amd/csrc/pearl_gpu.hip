@@ -57,7 +57,7 @@ struct pearl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int ndom = 0, k = 1;
-    int mx = 0;  // mixed precision (PEARL_MIXED)
+    int mx = 0;  // precision form: PEARL_FP64, PEARL_MIXED or PEARL_FP32_STENCIL (both FP32-storage forms are "mixed")
     std::vector<pearl_params> prm;
     std::vector<KP> kp;             // host copy of the parameter blocks
     std::vector<pearl_scalars> sc;  // host-side copy of the State scalars
@@ -175,6 +175,13 @@ void launch_cg_b(pearl_ctx* c, dim3 grid, const Bufs& B, int slot) {
     else by_k(std::false_type{});
 }
 
+// propose_phi in the context's form: FP32 stencil arithmetic only in the
+// PEARL_FP32_STENCIL form.
+void launch_propose(pearl_ctx* c, dim3 grid, const Bufs& B, int slot) {
+    if (c->mx == PEARL_FP32_STENCIL) PEARL_LAUNCH_K(c, k_propose, true, grid, B, slot);
+    else PEARL_LAUNCH_K(c, k_propose, false, grid, B, slot);
+}
+
 hipEvent_t get_event(pearl_ctx* c) {
     if (!c->free_events.empty()) {
         hipEvent_t e = c->free_events.back();
@@ -226,7 +233,7 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     if (with_state)
         ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, B, s); },
            [&] { hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B); });
-    ev(PEARL_KERNEL_PROPOSE, [&](int s) { PEARL_LAUNCH(c, k_propose, g, B, s); },
+    ev(PEARL_KERNEL_PROPOSE, [&](int s) { launch_propose(c, g, B, s); },
        [&] { hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B); });
     ev(PEARL_KERNEL_PREPARE, [&](int s) { PEARL_LAUNCH_P(c, k_prepare, g, B, s); },
        [&] { hipLaunchKernelGGL(k_prepare_fin, gf, tb, 0, st, B); });
@@ -285,7 +292,7 @@ void harvest(pearl_ctx* c) {
             if (r < 0) continue;
             const unsigned now = c->hS[d].runs[r];
             unsigned& seen = c->runs_seen[static_cast<size_t>(d) * NKIND + r];
-            c->bytes[kid] += kBytes[c->mx][kid] * c->kp[d].n * static_cast<double>(now - seen);
+            c->bytes[kid] += kBytes[c->mx ? 1 : 0][kid] * c->kp[d].n * static_cast<double>(now - seen);
             if (!c->prof) c->launches[kid] += now - seen;
             seen = now;
         }
@@ -506,6 +513,7 @@ KP make_kp(const pearl_params* params, double mu_reference, int k) {
     P.tau_min = params->tau[0];
     if (params->tau[1] < P.tau_min) P.tau_min = params->tau[1];
     if (params->tau[2] < P.tau_min) P.tau_min = params->tau[2];
+    P.f_tau_min = static_cast<float>(P.tau_min);
     const double eps = params->eps, sigma = params->sigma;
     P.obs = 16 * sigma / (std::acos(-1.) * std::acos(-1.) * eps);
     P.tr = params->triple / eps;
@@ -521,6 +529,17 @@ KP make_kp(const pearl_params* params, double mu_reference, int k) {
         P.es_p[k] = eps * s;
     }
     P.eps = eps;
+    P.f_obs = static_cast<float>(P.obs);
+    P.f_tr = static_cast<float>(P.tr);
+    for (int k = 0; k < 3; ++k) {
+        P.f_obs_p[k] = static_cast<float>(P.obs_p[k]);
+        P.f_fc_p[k] = static_cast<float>(P.fc_p[k]);
+        P.f_tau[k] = static_cast<float>(params->tau[k]);
+    }
+    P.f_rVm = static_cast<float>(1.0 / params->Vm);
+    P.f_rdx2 = static_cast<float>(P.rdx2);
+    P.f_rdy2 = static_cast<float>(P.rdy2);
+    P.f_eps = static_cast<float>(eps);
     P.sigma = sigma;
     P.Vm = params->Vm;
     auto psi = [&](int p, double mu) {
@@ -583,7 +602,8 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
     try {
         if (!out || count < 1 || !params || !mu_references) return PEARL_ERR_ARGUMENT;
         *out = nullptr;
-        if (precision != PEARL_FP64 && precision != PEARL_MIXED) return PEARL_ERR_ARGUMENT;
+        if (precision != PEARL_FP64 && precision != PEARL_MIXED && precision != PEARL_FP32_STENCIL)
+            return PEARL_ERR_ARGUMENT;
         for (int d = 0; d < count; ++d)
             if (!params_ok(&params[d])) return PEARL_ERR_ARGUMENT;
         // rows per tile; 4 measured best on the 4096x2048 workload (2, 8, 16
@@ -600,7 +620,7 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
         c->device = device;
         c->ndom = count;
         c->k = k;
-        c->mx = precision == PEARL_MIXED ? 1 : 0;
+        c->mx = precision;
         c->prm.assign(params, params + count);
         for (int d = 0; d < count; ++d) c->any_phase_only = c->any_phase_only || params[d].phase_only != 0;
         c->sc.assign(count, pearl_scalars{});
@@ -965,7 +985,7 @@ int pearl_propose_phi(pearl_ctx* c, double dt, double* phi_out, double* max_chan
         h.status = ST_OK;
         st = write_scalars(c);
         if (st) return st;
-        PEARL_LAUNCH(c, k_propose, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
+        launch_propose(c, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
         hipLaunchKernelGGL(k_propose_fin, dim3(static_cast<unsigned>(c->ndom)), dim3(TPB), 0, c->stream, c->B);
         st = read_scalars(c);
         if (st) return st;
@@ -1339,7 +1359,7 @@ int pearl_smooth(pearl_ctx* c, int max_steps, int min_steps, double energy_rtol,
         if (st) return st;
         if (!rep) return PEARL_ERR_ARGUMENT;
         std::memset(rep, 0, sizeof(*rep));
-        if (max_steps < 1 || min_steps < 0 || min_steps > max_steps || !(energy_rtol > 0) || !(max_phase_change > 0) ||
+        if (max_steps < 0 || min_steps < 0 || min_steps > max_steps || !(energy_rtol > 0) || !(max_phase_change > 0) ||
             max_phase_change > .2)
             return fail(c, PEARL_ERR_ARGUMENT, "invalid sharp-smoothing controls");
         st = read_scalars(c);
@@ -1355,6 +1375,7 @@ int pearl_smooth(pearl_ctx* c, int max_steps, int min_steps, double energy_rtol,
         st = read_scalars(c);
         if (st) return st;
         rep->energy_initial = h.sm_energy * P.dx * P.dy;
+        if (max_steps == 0) rep->converged = 1;  // the energy alone
         double previous = rep->energy_initial;
         const double eps = prm.eps, sigma = prm.sigma;
         const double base = .12 * std::min(prm.dx * prm.dx, prm.dy * prm.dy) / (eps * sigma);  // :884

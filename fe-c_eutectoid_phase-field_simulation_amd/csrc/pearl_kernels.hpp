@@ -55,6 +55,9 @@ struct KP {
     double sig_p[3], obs_p[3], fc_p[3], es_p[3];
     double eps, sigma, Vm;
     double tau[3], tau_min;
+    // FP32 copies for the FP32-stencil phase proposal (propose_z32): the
+    // constants above rounded once, and the reciprocals of Vm, dx^2, dy^2
+    float f_obs, f_tr, f_obs_p[3], f_fc_p[3], f_rVm, f_rdx2, f_rdy2, f_tau[3], f_tau_min, f_eps;
     double psiref[3];       // psi_a(mu_ref) - psi_gamma(mu_ref), psiref[2]=0  :797
     double mu_ref;
     // Controls (:767-773)
@@ -728,18 +731,90 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
     for (int a = 0; a < 3; ++a) z[a] = p[a] - scale * (force[a] - mean);
 }
 
+// The FP32-stencil form of propose_z (precision PEARL_FP32_STENCIL): the
+// same terms in the same order, with the force, its mean and the step scale
+// in FP32.  The chemical driving-force differences psi_b - psi_gamma are
+// formed in FP64 (each psi is ~1e4..1e5 J/mol, their differences ~1e1..1e2,
+// which FP32 would cancel to 1e-4 relative) and then rounded; the increment
+// scale*(force - mean) is rounded once and subtracted from the FP64 phases,
+// so the projection, the phase-increment test and the stored phases stay
+// FP64.  Constant divisions are multiplications by FP32 reciprocals.
+__device__ __forceinline__ void propose_z32(const KP& P, int y, const double (&pd)[3], const double (&qwd)[3],
+                                            const double (&qed)[3], const double (&qsd)[3], const double (&qnd)[3],
+                                            double m, double dt, double (&z)[3], const float* south,
+                                            float (&north)[3]) {
+    const double psi0 = psi(P, 2, m);
+    float dpsi[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) dpsi[b] = static_cast<float>(psi(P, b, m) - psi0);
+    float p[3], qw[3], qe[3], qs[3], qn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        p[a] = static_cast<float>(pd[a]);
+        qw[a] = static_cast<float>(qwd[a]);
+        qe[a] = static_cast<float>(qed[a]);
+        qs[a] = static_cast<float>(qsd[a]);
+        qn[a] = static_cast<float>(qnd[a]);
+    }
+    const float dtr[3] = {2.0f * p[1] * p[2], 2.0f * p[0] * p[2], 2.0f * p[0] * p[1]};
+    float force[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+        const float obs = P.pair ? P.f_obs_p[pair_of(a < a1 ? a : a1, a < a1 ? a1 : a)] * p[a1] +
+                                       P.f_obs_p[pair_of(a < a2 ? a : a2, a < a2 ? a2 : a)] * p[a2]
+                                 : P.f_obs * (p[a1] + p[a2]);
+        force[a] = obs + P.f_tr * p[a1] * p[a2];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float dh = dtr[a] + (b == a ? 6.0f * p[b] * (1.0f - p[b]) : 0.0f);
+            force[a] += dpsi[b] * dh * P.f_rVm;
+        }
+    }
+#pragma unroll
+    for (int dir = 0; dir < 4; ++dir) {
+        if ((dir == 2 && y == 0) || (dir == 3 && y == P.ny - 1)) continue;
+        const float(&q)[3] = dir == 0 ? qw : (dir == 1 ? qe : (dir == 2 ? qs : qn));
+        const float rdd = dir < 2 ? P.f_rdx2 : P.f_rdy2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 3; ++b) {
+                const float wv = (dir == 2 && south) ? -south[pair_of(a, b)]
+                                                     : P.f_fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]) * rdd;
+                if (dir == 3) north[pair_of(a, b)] = wv;
+                force[a] += wv * q[b];
+                force[b] -= wv * q[a];
+            }
+    }
+    const float mean = (force[0] + force[1] + force[2]) * (1.0f / 3.0f);
+    const float w0 = p[0] * p[1], w1 = p[0] * p[2], w2 = p[1] * p[2];
+    const float ws = w0 + w1 + w2;
+    const float tau = ws > 1e-30f ? (w0 * P.f_tau[0] + w1 * P.f_tau[1] + w2 * P.f_tau[2]) / ws : P.f_tau_min;
+    const float scale = static_cast<float>(dt) / (tau * P.f_eps);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) z[a] = pd[a] - static_cast<double>(scale * (force[a] - mean));
+}
+
 // propose_phi :961-1002 for one cell: p and the south/north phases from the
 // column registers, west/east loaded.  v[0] = max change, v[1] = projection
-// failure.
+// failure.  F32: the FP32-stencil form (propose_z32); the carried north face
+// terms are then FP32.
+template <bool F32, class W>
 __device__ __forceinline__ void propose_cell(const KP& P, const double* __restrict__ phi, int n, int i, int y,
                                              const Col& col, const double (&ps)[3], const double (&p)[3],
                                              const double (&pn)[3], double m, double dt, double* __restrict__ out,
-                                             double (&v)[2], const double* south, double (&north)[3]) {
+                                             double (&v)[2], const W* south, W (&north)[3]) {
     double qw[3], qe[3];
     load_phi(phi, n, i + col.dw, qw);
     load_phi(phi, n, i + col.de, qe);
-    double force[3], mean, scale, z[3];
-    propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north);
+    double z[3];
+    if constexpr (F32) {
+        propose_z32(P, y, p, qw, qe, ps, pn, m, dt, z, south, north);
+    } else {
+        double force[3], mean, scale;
+        propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north);
+    }
     if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
         v[0] = __builtin_inf();
 #pragma unroll
@@ -806,8 +881,10 @@ struct Phases {
 };
 
 // P1: propose_phi :961-1002, then the phase-increment test (:1131).
-template <int TY>
+// F32: the FP32-stencil form.
+template <int TY, bool F32>
 __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
+    using W = typename std::conditional<F32, float, double>::type;
     const Where w = where(B, -1);
     const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
@@ -820,7 +897,7 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     double* __restrict__ out = B.phi[cur ^ 1] + 3 * P.off;
     const double dt = S->dt;
     double v[2] = {0.0, 0.0};
-    double carry[3] = {0.0, 0.0, 0.0};
+    W carry[3] = {0, 0, 0};
     const Col col = column<TY>(P, w.lb, P.ny);
     march<TY, Phases>(
         P, col,
@@ -831,9 +908,9 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
         },
         [&](const int i, const int y, const Phases& s, const Phases& c, const Phases& nn) {
             // the previous cell of the march (y - 1) computed this cell's south terms
-            double north[3] = {0.0, 0.0, 0.0};
-            propose_cell(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
-                         y > col.y0 ? carry : nullptr, north);
+            W north[3] = {0, 0, 0};
+            propose_cell<F32, W>(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
+                                 y > col.y0 ? carry : nullptr, north);
 #pragma unroll
             for (int q = 0; q < 3; ++q) carry[q] = north[q];
         });

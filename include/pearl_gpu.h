@@ -57,6 +57,11 @@ extern "C" {
                                in FP32 and used exactly as stored; phi, c, mu, the operator's
                                coefficients (chi, K), x, r, Ap, every residual, dot product, the
                                carbon mass and the acceptance reductions in FP64 */
+#define PEARL_FP32_STENCIL 2 /* the mixed form with FP32 stencil arithmetic in the phase proposal:
+                               the force (obstacle, chemical, capillary face terms), its mean and
+                               the step scale in FP32 (the chemical driving-force differences
+                               formed in FP64 first); phi, the increment's application, the
+                               projection and everything after it as in PEARL_MIXED */
 
 /* Model and numerical parameters: the reference's Domain (:617), Thermo
  * (:76-91), PhaseParameters (:521-532) and Controls (:767-773). */
@@ -225,7 +230,10 @@ int pearl_valid_state(pearl_ctx* ctx, int* valid);
  * labels, :1798): projected gradient steps on interface_energy_only
  * (:822-841) with up to 24 halvings of the step per iteration.  The
  * interface energy is summed in the blocked order.  Carbon and mu are left
- * as they are.  Errors: PEARL_ERR_ARGUMENT ("invalid sharp-smoothing
+ * as they are.  max_steps = 0 (with min_steps = 0) only evaluates the
+ * interface energy of the current phases (energy_initial = energy_final,
+ * converged = 1), as initialize reports it for the diffuse profiles
+ * (:1802-1807).  Errors: PEARL_ERR_ARGUMENT ("invalid sharp-smoothing
  * controls"), PEARL_ERR_SMOOTHING, PEARL_ERR_PROJECTION. */
 int pearl_smooth(pearl_ctx* ctx, int max_steps, int min_steps, double energy_rtol, double max_phase_change,
                  pearl_smooth_report* report);

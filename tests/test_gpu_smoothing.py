@@ -80,6 +80,26 @@ def test_smoothing_rejects_bad_controls(pkg):
     n = 48 * 40
     with pkg.Solver(cfg.solver_params(), cfg.initial_mu) as s:
         s.set_state(phi0.reshape(3, n), np.zeros(n), np.zeros(n), [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
-        for args in [(0, 0, 1e-10, 0.02), (10, 11, 1e-10, 0.02), (10, 10, 0.0, 0.02), (10, 10, 1e-10, 0.3)]:
+        for args in [(-1, 0, 1e-10, 0.02), (0, 1, 1e-10, 0.02), (10, 11, 1e-10, 0.02), (10, 10, 0.0, 0.02),
+                     (10, 10, 1e-10, 0.3)]:
             with pytest.raises(pkg.PearlError, match="invalid sharp-smoothing controls"):
                 s.smooth(*args)
+
+
+def test_zero_steps_reports_the_interface_energy(pkg, O):
+    """max_steps = 0: the interface energy of the current phases alone, as
+    initialize reports it for the diffuse profiles (:1802-1807), equal to
+    the oracle's interface_energy_only in the blocked order (its smoothing's
+    initial energy); the phases are left as they are."""
+    cfg, phi0 = sharp_state(pkg, 48, 40, 2, 10, 10)
+    n = 48 * 40
+    with pkg.Solver(cfg.solver_params(), cfg.initial_mu) as s:
+        s.set_state(phi0.reshape(3, n), np.zeros(n), np.zeros(n), [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
+        rep = s.smooth(0, 0, 1e-10, 0.02)
+        k = s.layout()[1]
+        assert np.array_equal(s.get_state()[0].reshape(3, n), phi0.reshape(3, n))
+    assert rep["steps"] == 0 and rep["converged"] == 1
+    assert rep["energy_initial"] == rep["energy_final"] > 0
+    Pb = O.params_from_dict(cfg.solver_params(), red_mode=O.RED_BLOCKED, red_k=k)
+    rc, orep = O.smooth(Pb, np.ascontiguousarray(phi0, dtype=np.float64).copy(), 1, 1, 1e-10, 0.02)
+    assert rc == 0 and rep["energy_initial"] == orep[2]
