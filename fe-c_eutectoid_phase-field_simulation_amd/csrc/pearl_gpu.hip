@@ -37,14 +37,13 @@ const char* kKernelNames[PEARL_KERNEL_COUNT] = {
 
 // Algorithmic bytes per cell of each kernel (each logical array counted
 // once per pass): propose R phi(3),mu W phi'(3); prepare R phi'(3),c,mu
-// W chi,icpt,K,rhs,x,diag,r,z; apply R z,p_old,K,chi W p,Ap; update
+// W chi,K,rhs,x,diag,r,z; apply R z,p_old,K,chi W p,Ap; update
 // R x,p,r,Ap,diag W x,r,z; true residual R x,K,chi,rhs,diag W r,z;
-// finalize+accept R phi'(3),x,K,c,icpt,chi W c',mu'; state reduce
-// R phi(3),c,mu.  Plain form: 8 B per value.  Mixed form: diag, z and p are
-// 4 B.
+// finalize+accept R phi'(3),x,K,c W c',mu'; state reduce R phi(3),c,mu.
+// Plain form: 8 B per value.  Mixed form: diag, z and p are 4 B.
 const double kBytes[2][PEARL_KERNEL_COUNT] = {
-    {56, 104, 48, 64, 56, 0, 80, 40, 0, 0},
-    {56, 96, 36, 52, 48, 0, 80, 40, 0, 0},
+    {56, 96, 48, 64, 56, 0, 64, 40, 0, 0},
+    {56, 88, 36, 52, 48, 0, 64, 40, 0, 0},
 };
 
 // Device run counter of each kernel id (-1: none).
@@ -653,7 +652,7 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
         // power-of-two domain the bare spacing would put the same cell of every
         // plane on the same cache set and memory channel.
         const size_t plane = static_cast<size_t>(off) + 576;
-        const size_t total = plane * (3 * 2 + 2 + 2 + 11) + static_cast<size_t>(poff) + 64;
+        const size_t total = plane * (3 * 2 + 2 + 2 + 10) + static_cast<size_t>(poff) + 64;
         auto bail = [&](int code) {
             pearl_destroy(c);
             (void)hipGetLastError();  // the failed HIP call's error is reported here only
@@ -673,7 +672,7 @@ int pearl_batch_create(int count, const pearl_params* params, const double* mu_r
             c->B.mu[b] = q;
             q += plane;
         }
-        double** work[] = {&c->B.chi, &c->B.icpt, &c->B.K, &c->B.rhs, &c->B.x, &c->B.diag,
+        double** work[] = {&c->B.chi, &c->B.K, &c->B.rhs, &c->B.x, &c->B.diag,
                            &c->B.r,   &c->B.z,    &c->B.p0, &c->B.p1, &c->B.Ap};
         for (double** w : work) {
             *w = q;

@@ -376,10 +376,11 @@ __device__ __forceinline__ double mixture_K(const KP& P, const double (&phi)[3])
 // Per-cell acceptance quantities of valid_state (:1098-1114) and the local
 // energy of reduced_energy (:799-819): the cell term, then the east and
 // (below the top row) north face terms in the reference's order.  p, qe, qn:
-// phases of the cell and of its east and north neighbours.
+// phases of the cell and of its east and north neighbours.  `known`: the
+// cell's mixture when the caller has already formed it (the same value).
 __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], const double (&qe)[3],
                                             const double (&qn)[3], bool north, double c, double mu, bool& good,
-                                            double& energy, bool& mix_ok) {
+                                            double& energy, bool& mix_ok, const Mix* known = nullptr) {
     double sum = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -404,7 +405,8 @@ __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], c
     for (int a = 0; a < 3; ++a) chem += h[a] * P.psiref[a];
     if (!P.phase_only) {
         Mix m;
-        if (!mixture(P, p, m)) mix_ok = false;
+        if (known) m = *known;
+        else if (!mixture(P, p, m)) mix_ok = false;
         const double dc = c - composition(P.mu_ref, m);
         chem += .5 * dc * dc / m.s;
     }
@@ -446,7 +448,7 @@ struct Bufs {
     double* phi[2];   // domain d: phi[b] + 3*off + a*n + i
     double* c[2];
     double* mu[2];
-    double *chi, *icpt, *K, *rhs, *x, *diag, *r, *z, *p0, *p1, *Ap;
+    double *chi, *K, *rhs, *x, *diag, *r, *z, *p0, *p1, *Ap;
     double* part;     // reduction partials, domain d at part + poff
     const KP* kp;
     DevScal* S;
@@ -988,7 +990,6 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     const double* __restrict__ c_old = B.c[cur] + P.off;
     const double* __restrict__ mu_old = B.mu[cur] + P.off;
     double* __restrict__ chi = B.chi + P.off;
-    double* __restrict__ icpt = B.icpt + P.off;
     double* __restrict__ K = B.K + P.off;
     double* __restrict__ rhs = B.rhs + P.off;
     double* __restrict__ x = B.x + P.off;
@@ -998,26 +999,49 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     // LDS tile: x = mu_old - mu_ref and K = mixture's K of the trial phases,
-    // for the tile and its halo (each formed once per block)
+    // for the tile and its halo (each formed once per block).  The thread
+    // that stages a column's centre rows keeps their whole mixture (chi and
+    // c0; K is the same expression's value) in registers, so the trial
+    // phases are read once per cell.
     __shared__ TileLds<TY> L;
-    stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
-        double p[3];
-        load_phi(phi_t, n, j, p);
-        L.a[r][cc] = mu_old[j] - P.mu_ref;
-        L.b[r][cc] = mixture_K(P, p);
-    });
+    double mix_s[TY], mix_c0[TY];
+    const int t = static_cast<int>(threadIdx.x);
+    auto stage_column = [&](const int cc, int gx, const bool centre) {
+        gx = gx < 0 ? gx + P.nx : (gx >= P.nx ? gx % P.nx : gx);
+#pragma unroll
+        for (int rw = 0; rw < TY + 2; ++rw) {
+            int y = col.y0 - 1 + rw;
+            y = y < 0 ? 0 : (y > P.ny - 1 ? P.ny - 1 : y);
+            const int j = y * P.nx + gx;
+            double p[3];
+            load_phi(phi_t, n, j, p);
+            L.a[rw][cc] = mu_old[j] - P.mu_ref;
+            if (centre && rw >= 1 && rw <= TY) {
+                Mix m;
+                const bool ok = mixture(P, p, m);
+                if (!ok && col.on && col.y0 + rw - 1 < col.y1) v[3] = 1.0;
+                mix_s[rw - 1] = m.s;
+                mix_c0[rw - 1] = m.c0;
+                L.b[rw][cc] = m.K;
+            } else {
+                L.b[rw][cc] = mixture_K(P, p);
+            }
+        }
+    };
+    stage_column(t + 1, col.x, true);
+    if (t == 0) stage_column(0, col.x - 1, false);
+    if (t == TPB - 1) stage_column(TPB + 1, col.x + 1, false);
     __syncthreads();
     for_tile<TY>(P, col, [&](const int i, const int y, const int row, const int cc) {
-        double p[3];
-        load_phi(phi_t, n, i, p);
         Mix m;
-        if (!mixture(P, p, m)) v[3] = 1.0;
+        m.s = mix_s[row - 1];
+        m.c0 = mix_c0[row - 1];
+        m.K = 0.0;  // not used: composition needs chi and c0
         const double ic = composition(P.mu_ref, m);
         const double b = ld_once(c_old + i) - ic;
         const double ch = m.s;
         const double kc = L.b[row][cc], xc = L.a[row][cc];
         st_once(chi + i, ch);
-        st_once(icpt + i, ic);
         st_once(K + i, kc);
         st_once(rhs + i, b);
         st_once(x + i, xc);
@@ -1308,8 +1332,6 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     const int cur = S->cur;
     const double dt = S->dt;
     const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
-    const double* __restrict__ chi = B.chi + P.off;
-    const double* __restrict__ icpt = B.icpt + P.off;
     const double* __restrict__ K = B.K + P.off;
     const double* __restrict__ x = B.x + P.off;
     const double* __restrict__ c_old = B.c[cur] + P.off;
@@ -1342,7 +1364,11 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
             const double change = dt * (wf[0] * (L.a[r][cc - 1] - xi) + wf[1] * (L.a[r][cc + 1] - xi) +
                                         wf[2] * (L.a[r - 1][cc] - xi) + wf[3] * (L.a[r + 1][cc] - xi));
             const double cn = ld_once(c_old + i) + change;
-            const double ic = ld_once(icpt + i), ch = ld_once(chi + i);
+            // the intercept and susceptibility prepare formed from the same
+            // trial phases: the same expressions, hence the same values
+            Mix m;
+            if (!mixture(P, c.p, m)) mix_ok = false;
+            const double ic = composition(P.mu_ref, m), ch = m.s;
             v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
             const double mn = P.mu_ref + (cn - ic) / ch;
             st_once(c_new + i, cn);
@@ -1350,7 +1376,7 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
             double qe[3];
             load_phi(phi_t, n, i + col.de, qe);
             double e;
-            accept_vals(P, c.p, qe, nn.p, y != P.ny - 1, cn, mn, good, e, mix_ok);
+            accept_vals(P, c.p, qe, nn.p, y != P.ny - 1, cn, mn, good, e, mix_ok, &m);
             v[2] += cn;
             v[3] += e;
         });
