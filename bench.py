@@ -630,7 +630,9 @@ def mixed_beside(pkg, cfg, phi, c, mu, initial_carbon, args, have_torch, torch, 
         out["lockstep"]["workload"] = f"{nx}x{ny}, {args.pairs} pairs, {args.undercooling} K (this workload)"
         nx2, ny2, pairs2, uc2 = 8192, 4096, 64, 20.0
         log(f"mixed form: lockstep against FP64 on {nx2}x{ny2}, {pairs2} pairs, {uc2} K, {n_lock} steps")
-        _, cfg2, phi2, c2, mu2 = build_workload(pkg, nx2, ny2, pairs2, ny2 // 8, args.seed, uc2, tmp)
+        # a directory of its own: the workload's input file must not replace this run's
+        _, cfg2, phi2, c2, mu2 = build_workload(pkg, nx2, ny2, pairs2, ny2 // 8, args.seed, uc2,
+                                                tempfile.mkdtemp(dir=tmp))
         with pkg.Solver(cfg2.solver_params(), cfg2.initial_mu) as z:
             z.set_state(phi2, c2, mu2, [0.0, cfg2.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
             m2 = z.total_carbon()
