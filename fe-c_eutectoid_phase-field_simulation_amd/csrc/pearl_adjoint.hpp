@@ -211,7 +211,8 @@ __global__ __launch_bounds__(TPB) void k_adj_propose_a(Bufs B, Adj A) {
             const double m = A.mu0[i];
             double force[3], mean, scale, z[3];
             double north[3];
-            propose_z(P, y, p, qw, qe, ps.p, pn.p, m, dt, force, mean, scale, z, nullptr, north);
+            DivConst div;
+            propose_z(P, y, p, qw, qe, ps.p, pn.p, m, dt, force, mean, scale, z, nullptr, north, div);
             double gt[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) gt[a] = A.gphit[a * n + i];

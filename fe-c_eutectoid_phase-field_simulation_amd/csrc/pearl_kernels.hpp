@@ -144,13 +144,54 @@ __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b
 // returns |q2| with the sign of a (d > 0), which also gives a zero
 // numerator its sign back.  In range: one multiply, four FMAs and the
 // fix-up instead of the scaled reciprocal iteration of a general division.
-__device__ __forceinline__ double div_const(double a, double d, double y) {
-    const int e = __builtin_amdgcn_frexp_exp(a);
-    if (__builtin_expect(static_cast<unsigned>(e + 817) > 1567u, 0)) return a / d;
+__device__ __forceinline__ double div_markstein(double a, double d, double y) {
     const double q0 = a * y;
     const double q1 = __builtin_fma(__builtin_fma(-q0, d, a), y, q0);
     const double q2 = __builtin_fma(__builtin_fma(-q1, d, a), y, q1);
     return __builtin_amdgcn_div_fixup(q2, d, a);
+}
+
+__device__ __forceinline__ double div_const(double a, double d, double y) {
+    const int e = __builtin_amdgcn_frexp_exp(a);
+    if (__builtin_expect(static_cast<unsigned>(e + 817) > 1567u, 0)) return a / d;
+    return div_markstein(a, d, y);
+}
+
+// The same quotients for a run of constant divisions whose numerators are
+// range-checked once, after the run: DivRun takes div_markstein for every
+// quotient and keeps the least and greatest frexp exponent of the
+// numerators; where ok() fails the caller computes the run again with
+// DivIEEE, whose IEEE quotients equal div_markstein's wherever the range
+// holds.  In a run of n divisions this costs one exponent, one min and one
+// max per division and a single branch, instead of a branch per division.
+struct DivRun {
+    int lo = 0, hi = 0;
+    __device__ __forceinline__ double operator()(double a, double d, double y) {
+        const int e = __builtin_amdgcn_frexp_exp(a);
+        lo = e < lo ? e : lo;
+        hi = e > hi ? e : hi;
+        return div_markstein(a, d, y);
+    }
+    __device__ __forceinline__ bool ok() const { return lo >= -817 && hi <= 750; }
+};
+struct DivIEEE {
+    __device__ __forceinline__ double operator()(double a, double d, double) const { return a / d; }
+};
+// div_const itself as a policy (each quotient range-checked on its own).
+struct DivConst {
+    __device__ __forceinline__ double operator()(double a, double d, double y) const { return div_const(a, d, y); }
+};
+
+// f(div) once with DivRun, and again with DivIEEE where the run's range
+// check fails; f writes its results, so the second call replaces them.
+template <class F>
+__device__ __forceinline__ void checked_run(F&& f) {
+    DivRun fast;
+    f(fast);
+    if (__builtin_expect(!fast.ok(), 0)) {
+        DivIEEE ieee;
+        f(ieee);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -316,7 +357,8 @@ __device__ __forceinline__ double local_tau(const KP& P, const double (&p)[3]) {
 }
 
 // project_simplex :491-509; false where the reference throws.
-__device__ __forceinline__ bool project_simplex(const double (&z)[3], double r3, double (&out)[3]) {
+template <class Div>
+__device__ __forceinline__ bool project_simplex(const double (&z)[3], double r3, double (&out)[3], Div& div) {
     double s0 = z[0], s1 = z[1], s2 = z[2], t;
     if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
     if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
@@ -328,10 +370,15 @@ __device__ __forceinline__ bool project_simplex(const double (&z)[3], double r3,
     sum += s1;
     { const double cand = (sum - 1.0) / 2.0; if (s1 > cand) { active = 2; threshold = cand; } }
     sum += s2;
-    { const double cand = div_const(sum - 1.0, 3.0, r3); if (s2 > cand) { active = 3; threshold = cand; } }
+    { const double cand = div(sum - 1.0, 3.0, r3); if (s2 > cand) { active = 3; threshold = cand; } }
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = dmaxd(0.0, z[a] - threshold);
     return active != 0;
+}
+
+__device__ __forceinline__ bool project_simplex(const double (&z)[3], double r3, double (&out)[3]) {
+    DivConst div;
+    return project_simplex(z, r3, out, div);
 }
 
 // Stencil neighbours of cell i in row y: x periodic; at the y edges the
@@ -351,10 +398,12 @@ __device__ __forceinline__ void load_phi(const double* __restrict__ phi, int n, 
 // neighbours (w, e, s, n).
 __device__ __forceinline__ void face_weights(const KP& P, int y, double k, double kw, double ke, double ks, double kn,
                                              double (&w)[4]) {
-    w[0] = div_const(.5 * (k + kw), P.dx2, P.rdx2);
-    w[1] = div_const(.5 * (k + ke), P.dx2, P.rdx2);
-    w[2] = div_const(.5 * (k + ks), P.dy2, P.rdy2);
-    w[3] = div_const(.5 * (k + kn), P.dy2, P.rdy2);
+    checked_run([&](auto& div) {
+        w[0] = div(.5 * (k + kw), P.dx2, P.rdx2);
+        w[1] = div(.5 * (k + ke), P.dx2, P.rdx2);
+        w[2] = div(.5 * (k + ks), P.dy2, P.rdy2);
+        w[3] = div(.5 * (k + kn), P.dy2, P.rdy2);
+    });
     if (y == 0) w[2] = 0;
     if (y == P.ny - 1) w[3] = 0;
 }
@@ -378,9 +427,11 @@ __device__ __forceinline__ double mixture_K(const KP& P, const double (&phi)[3])
 // (below the top row) north face terms in the reference's order.  p, qe, qn:
 // phases of the cell and of its east and north neighbours.  `known`: the
 // cell's mixture when the caller has already formed it (the same value).
-__device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], const double (&qe)[3],
-                                            const double (&qn)[3], bool north, double c, double mu, bool& good,
-                                            double& energy, bool& mix_ok, const Mix* known = nullptr) {
+// div: the constant-division policy; accept_vals below runs it checked.
+template <class Div>
+__device__ __forceinline__ void accept_vals_d(const KP& P, const double (&p)[3], const double (&qe)[3],
+                                              const double (&qn)[3], bool north, double c, double mu, bool& good,
+                                              double& energy, bool& mix_ok, const Mix* known, Div& div) {
     double sum = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -394,7 +445,7 @@ __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], c
 #pragma unroll
         for (int a = 0; a < 3; ++a)
             if (p[a] > .01) {
-                const double cp = div_const(mu - P.B[a], P.twoA[a], P.rtwoA[a]);
+                const double cp = div(mu - P.B[a], P.twoA[a], P.rtwoA[a]);
                 good = good && cp >= -1e-10 && cp <= 1 + 1e-10;
             }
     }
@@ -410,12 +461,12 @@ __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], c
         const double dc = c - composition(P.mu_ref, m);
         chem += .5 * dc * dc / m.s;
     }
-    double v = div_const(chem, P.Vm, P.rVm) + obstacle_energy(P, p) + P.tr * p[0] * p[1] * p[2];
+    double v = div(chem, P.Vm, P.rVm) + obstacle_energy(P, p) + P.tr * p[0] * p[1] * p[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int bb = a + 1; bb < 3; ++bb) {
-            const double z = div_const(p[a] * qe[bb] - qe[a] * p[bb], P.dx, P.rdx);
+            const double z = div(p[a] * qe[bb] - qe[a] * p[bb], P.dx, P.rdx);
             v += P.es_p[pair_of(a, bb)] * z * z;
         }
     if (north) {
@@ -423,11 +474,26 @@ __device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], c
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int bb = a + 1; bb < 3; ++bb) {
-                const double z = div_const(p[a] * qn[bb] - qn[a] * p[bb], P.dy, P.rdy);
+                const double z = div(p[a] * qn[bb] - qn[a] * p[bb], P.dy, P.rdy);
                 v += P.es_p[pair_of(a, bb)] * z * z;
             }
     }
     energy = v;
+}
+
+// accept_vals_d with the cell's constant divisions as one checked run; good
+// and mix_ok are cleared where the cell fails.
+__device__ __forceinline__ void accept_vals(const KP& P, const double (&p)[3], const double (&qe)[3],
+                                            const double (&qn)[3], bool north, double c, double mu, bool& good,
+                                            double& energy, bool& mix_ok, const Mix* known = nullptr) {
+    bool g, mo;
+    checked_run([&](auto& div) {
+        g = true;
+        mo = true;
+        accept_vals_d(P, p, qe, qn, north, c, mu, g, energy, mo, known, div);
+    });
+    good = good && g;
+    mix_ok = mix_ok && mo;
 }
 
 __device__ __forceinline__ void accept_cell(const KP& P, const double* __restrict__ phi, int i, const Nbr& b,
@@ -690,11 +756,12 @@ __global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
 // product and the constant division are all sign-symmetric in IEEE
 // arithmetic), so a column march passes the north terms down: `south`, when
 // given, holds the southern neighbour's north terms, and `north` receives
-// this cell's.
+// this cell's.  div: the constant-division policy (DivRun or DivIEEE).
+template <class Div>
 __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[3], const double (&qw)[3],
                                           const double (&qe)[3], const double (&qs)[3], const double (&qn)[3],
                                           double m, double dt, double (&force)[3], double& mean, double& scale,
-                                          double (&z)[3], const double* south, double (&north)[3]) {
+                                          double (&z)[3], const double* south, double (&north)[3], Div& div) {
     const double psi0 = psi(P, 2, m);
     const double dtr[3] = {2.0 * p[1] * p[2], 2.0 * p[0] * p[2], 2.0 * p[0] * p[1]};
     double dpsi[3];
@@ -707,7 +774,11 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const double dh = dtr[a] + (b == a ? 6.0 * p[b] * (1.0 - p[b]) : 0.0);
-            force[a] += div_const(dpsi[b] * dh, P.Vm, P.rVm);
+            // dpsi[2] = psi0 - psi0 is +0 (NaN if psi0 is not finite), so its
+            // product is a signed zero or NaN, which divided by the positive
+            // Vm is itself: that quotient needs no division.
+            const double t = dpsi[b] * dh;
+            force[a] += b == 2 ? t : div(t, P.Vm, P.rVm);
         }
     }
 #pragma unroll
@@ -721,13 +792,13 @@ __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[
             for (int b = a + 1; b < 3; ++b) {
                 const double wv = (dir == 2 && south)
                                       ? -south[pair_of(a, b)]
-                                      : div_const(P.fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
+                                      : div(P.fc_p[pair_of(a, b)] * (p[a] * q[b] - q[a] * p[b]), dd, rdd);
                 if (dir == 3) north[pair_of(a, b)] = wv;
                 force[a] += wv * q[b];
                 force[b] -= wv * q[a];
             }
     }
-    mean = div_const(force[0] + force[1] + force[2], 3.0, P.r3);
+    mean = div(force[0] + force[1] + force[2], 3.0, P.r3);
     scale = dt / (local_tau(P, p) * P.eps);
 #pragma unroll
     for (int a = 0; a < 3; ++a) z[a] = p[a] - scale * (force[a] - mean);
@@ -810,21 +881,28 @@ __device__ __forceinline__ void propose_cell(const KP& P, const double* __restri
     double qw[3], qe[3];
     load_phi(phi, n, i + col.dw, qw);
     load_phi(phi, n, i + col.de, qe);
-    double z[3];
+    double z[3], zp[3];
+    bool finite, projected = true;
     if constexpr (F32) {
         propose_z32(P, y, p, qw, qe, ps, pn, m, dt, z, south, north);
+        finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
+        if (finite) projected = project_simplex(z, P.r3, zp);
     } else {
-        double force[3], mean, scale;
-        propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north);
+        // the cell's constant divisions as one checked run
+        checked_run([&](auto& div) {
+            double force[3], mean, scale;
+            propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north, div);
+            finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
+            projected = !finite || project_simplex(z, P.r3, zp, div);
+        });
     }
-    if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
+    if (!finite) {
         v[0] = __builtin_inf();
 #pragma unroll
         for (int a = 0; a < 3; ++a) st_once(out + a * n + i, p[a]);
         return;
     }
-    double zp[3];
-    if (!project_simplex(z, P.r3, zp)) v[1] = 1.0;
+    if (!projected) v[1] = 1.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
 #pragma unroll
