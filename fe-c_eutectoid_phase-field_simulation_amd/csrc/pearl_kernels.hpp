@@ -751,12 +751,20 @@ __global__ __launch_bounds__(TPB) void k_state_fin(Bufs B) {
 // the step scale and z = phi - scale*(force - mean), in the reference's
 // operation order.  p and its west, east, south, north neighbours qw, qe,
 // qs, qn (the south/north terms are skipped at the y edges).
-// The face term w of a pair on a cell's south face is exactly the negation
-// of the term on its southern neighbour's north face (the difference, the
-// product and the constant division are all sign-symmetric in IEEE
-// arithmetic), so a column march passes the north terms down: `south`, when
-// given, holds the southern neighbour's north terms, and `north` receives
-// this cell's.  div: the constant-division policy (DivRun or DivIEEE).
+// The face term w of a pair on a cell's south face is the negation of the
+// term on its southern neighbour's north face (the difference, the product
+// and the constant division are all sign-symmetric in IEEE arithmetic),
+// except that where the two products are equal both differences are +0, so
+// the negation is -0 where the direct term is +0.  That sign never shows:
+// adding or subtracting a signed zero leaves the force unchanged unless the
+// force is -0, and it never is.  It starts as obs (phi_a1 + phi_a2) (or
+// obs_a,a1 phi_a1 + obs_a,a2 phi_a2) plus tr phi_a1 phi_a2 with obs > 0 and
+// tr >= 0; the first part is -0 only when both phases are -0 or negative,
+// which makes the second +0 or positive; and a sum or difference is -0 only
+// when the force already is.  So a column march passes the north terms
+// down: `south`, when given, holds the southern neighbour's north terms,
+// and `north` receives this cell's.  div: the constant-division policy
+// (DivRun or DivIEEE).
 template <class Div>
 __device__ __forceinline__ void propose_z(const KP& P, int y, const double (&p)[3], const double (&qw)[3],
                                           const double (&qe)[3], const double (&qs)[3], const double (&qn)[3],
