@@ -120,45 +120,19 @@ COMPOSITION_MU, COMPOSITION_MEAN = 0, 1
 
 
 def fill_ops(cfg):
-    """The sharp labelling of `cfg` (a config.Config) as pearl_fill
-    operations: its FILL lines for initialization = microsim, or the two
-    boxes of sharp_smooth (theta in the first theta_columns, alpha in the
-    rest, of the rows whose centre (gy + 0.5) dy lies below seed_height,
-    pearl_serial.cpp:1777-1786).  The directions are evaluated as the
-    reference does, cos(angle*pi/180.0) with pi = 3.14159265358979323846
-    (:1396, :1423-1424, :1440-1441, :1487-1488)."""
-    import math
-    pi = 3.14159265358979323846
-    ops = []
-    if cfg.initialization == "microsim":
-        for method, params in cfg.filling:
-            f = Fill()
-            f.method = FILL_METHODS.index(method)
-            for i, v in enumerate(params):
-                f.p[i] = float(v)
-            if method in ("FILLYJUNCTION2D", "FILLYJUNCTIONLAMELLAE2D"):
-                angles = [(k, params[6 + k]) for k in range(3)]
-            elif method in ("FILLADHEREDPEARLITEARM2D", "FILLPEARLITEELLIPSE2D"):
-                angles = [(0, params[2])]
-            else:
-                angles = []
-            for k, a in angles:
-                ang = float(a) * pi / 180.0
-                f.dir[k][0], f.dir[k][1] = math.cos(ang), math.sin(ang)
-            ops.append(f)
-    elif cfg.initialization == "sharp_smooth":
-        rows = 0
-        while rows < cfg.ny and (rows + 0.5) * cfg.dy < cfg.seed_height:
-            rows += 1
-        for phase, x0, x1 in ((1, 0, cfg.theta_columns - 1), (0, cfg.theta_columns, cfg.nx - 1)):
-            f = Fill()
-            f.method = 0
-            f.p[0], f.p[1], f.p[2], f.p[4], f.p[5] = phase, x0, 0, x1, rows - 1
-            ops.append(f)
-    else:
-        raise ValueError(f"initialization = {cfg.initialization} has no sharp labels")
-    arr = (Fill * max(1, len(ops)))(*ops)
-    return arr, len(ops)
+    """The sharp labelling of `cfg` (a config.Config read from its input
+    file) as pearl_fill operations, formed by the input reader of the
+    drop-in program (libpearl_input.so): the admitted FILL lines for
+    initialization = microsim, with each direction the reference's
+    cos(angle*pi/180.0) (:1396), or the two boxes of sharp_smooth (theta in
+    the first theta_columns, alpha in the rest, of the rows whose centre
+    (gy + 0.5) dy lies below seed_height, pearl_serial.cpp:1777-1786).
+    Returns (array of Fill, count)."""
+    from . import config
+    try:
+        return config.sharp_label_ops(cfg, Fill)
+    except config.ConfigError as e:
+        raise ValueError(str(e)) from None
 
 
 EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state", "pearl_get_state",

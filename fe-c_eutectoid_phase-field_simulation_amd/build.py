@@ -34,6 +34,21 @@ def sources():
 
 HDF5 = os.environ.get("PEARL_HDF5_PREFIX", "/opt/conda")
 EXE = os.path.join(PKG, "pearl_gpu")
+INPUT_LIB = os.path.join(PKG, "libpearl_input.so")
+
+
+def build_input(verbose: bool = False) -> str:
+    """Compile libpearl_input.so (C++17, host only): the drop-in program's
+    input reader behind include/pearl_input.h, which the Python binding
+    calls instead of parsing input files itself."""
+    tmp = INPUT_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC", "-shared", f"-I{INCLUDE}",
+           f"-I{os.path.join(CSRC, 'host')}", os.path.join(CSRC, "host", "input_capi.cpp"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, INPUT_LIB)
+    return INPUT_LIB
 
 
 def build_host(verbose: bool = False) -> str | None:
@@ -64,7 +79,8 @@ def build_host(verbose: bool = False) -> str | None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile libpearl_gpu.so and the host program (always recompiles)."""
+    """Compile libpearl_gpu.so, libpearl_input.so and the host program
+    (always recompiles)."""
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     tmp = LIB + ".tmp"
@@ -73,6 +89,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_input(verbose)
     build_host(verbose)
     return LIB
 

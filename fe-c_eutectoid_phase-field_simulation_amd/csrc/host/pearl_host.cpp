@@ -304,26 +304,6 @@ void diffuse_phases(const Settings& c, HostState& st) {
         }
 }
 
-// sharp_smooth's labels as two FILLCUBE boxes: theta in the first
-// theta_columns columns, alpha in the rest, for the rows whose centre lies
-// below seed_height (:1777-1786).
-std::vector<pearl_fill> sharp_smooth_boxes(const Settings& c) {
-    int rows = 0;
-    while (rows < c.ny && (rows + .5) * c.dy < c.seed_height) ++rows;
-    std::vector<pearl_fill> boxes(2);
-    const double spans[2][3] = {{1, 0, static_cast<double>(c.theta_columns - 1)},
-                                {0, static_cast<double>(c.theta_columns), static_cast<double>(c.nx - 1)}};
-    for (int b = 0; b < 2; ++b) {
-        boxes[b] = pearl_fill{};
-        boxes[b].method = PEARL_FILL_CUBE;
-        boxes[b].p[0] = spans[b][0];
-        boxes[b].p[1] = spans[b][1];
-        boxes[b].p[4] = spans[b][2];
-        boxes[b].p[5] = rows - 1;
-    }
-    return boxes;
-}
-
 // ---------------------------------------------------------------------------
 // The run
 // ---------------------------------------------------------------------------
@@ -383,8 +363,7 @@ private:
         InitReport rep;
         const pearl_scalars zero{};
         if (cfg_.sharp_labels()) {
-            const std::vector<pearl_fill> ops =
-                cfg_.initialization == "microsim" ? cfg_.fills : sharp_smooth_boxes(cfg_);
+            const std::vector<pearl_fill> ops = sharp_label_ops(cfg_);
             init.check(pearl_fill_phases(init.get(), ops.data(), static_cast<int>(ops.size())));
             init.download_phases(state_);
             write_phases(out_ / "init_sharp.h5", state_, cfg_, "init_sharp", 0);

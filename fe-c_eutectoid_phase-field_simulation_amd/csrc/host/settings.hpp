@@ -551,6 +551,30 @@ inline const std::vector<Item>& sections() {
 
 }  // namespace schema
 
+// The sharp labels of an input as pearl_fill operations: the admitted FILL
+// lines of initialization = microsim (:1772-1776), or sharp_smooth's two
+// FILLCUBE boxes, theta in the first theta_columns columns and alpha in the
+// rest, over the rows whose centre lies below seed_height (:1777-1786).
+inline std::vector<pearl_fill> sharp_label_ops(const Settings& c) {
+    if (c.initialization == "microsim") return c.fills;
+    if (c.initialization != "sharp_smooth")
+        throw std::invalid_argument("initialization = " + c.initialization + " has no sharp labels");
+    int rows = 0;
+    while (rows < c.ny && (rows + .5) * c.dy < c.seed_height) ++rows;
+    std::vector<pearl_fill> boxes(2);
+    const double spans[2][3] = {{1, 0, static_cast<double>(c.theta_columns - 1)},
+                                {0, static_cast<double>(c.theta_columns), static_cast<double>(c.nx - 1)}};
+    for (int b = 0; b < 2; ++b) {
+        boxes[b] = pearl_fill{};
+        boxes[b].method = PEARL_FILL_CUBE;
+        boxes[b].p[0] = spans[b][0];
+        boxes[b].p[1] = spans[b][1];
+        boxes[b].p[4] = spans[b][2];
+        boxes[b].p[5] = rows - 1;
+    }
+    return boxes;
+}
+
 inline Settings read_settings(const std::string& path) {
     InputFile in = InputFile::read(path);
     Settings s;

@@ -28,6 +28,19 @@ def test_library_exports_every_declared_symbol():
     assert sorted(pkg.EXPORTS) == declared_symbols()
 
 
+def test_input_library_exports_every_declared_symbol():
+    """libpearl_input.so (the input reader) exports what include/pearl_input.h declares."""
+    load_pkg()
+    import importlib
+    cfg = importlib.import_module("pearl_amd.config")
+    text = open(os.path.join(ROOT, "include", "pearl_input.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    syms = sorted(set(re.findall(r"\b(pearl_[a-z_0-9]+)\s*\(", text)))
+    assert syms == ["pearl_input_fills", "pearl_linear_gp_law", "pearl_phase_diagram_law", "pearl_read_input"]
+    lib = ctypes.CDLL(cfg.INPUT_LIB)
+    assert not [s for s in syms if not hasattr(lib, s)]
+
+
 def test_error_messages_match_reference_text():
     pkg = load_pkg()
     assert pkg.WHY == ["phase increment", "linear/closure", "composition/phase bounds", "mass conservation",

@@ -1,5 +1,6 @@
-"""The host-side input parser and temperature laws against the reference
-parser (bitwise) and against the published 10 K values."""
+"""The input reader (the drop-in program's, bound by config.py) and its
+temperature laws against the reference parser (bitwise) and against the
+published 10 K values."""
 import glob
 import os
 import tempfile
@@ -42,9 +43,14 @@ def test_phase_diagram_law_published_values():
     assert out[2] == ref[2]
     # zero undercooling preserves every supplied bit
     assert C.at_phase_diagram_undercooling(ref, ceq, slopes, 0.0) == ref
-    # initial mu from gamma_carbon (NEW_FREE_ENERGY.md: 31345.33897721396 J/mol)
-    mu = C._fma(2.0 * ref[2][0], ceq[2], ref[2][1])
+    # initial mu from gamma_carbon (NEW_FREE_ENERGY.md: 31345.33897721396 J/mol),
+    # mu = fma(2A, c, B) rounded once
+    from fractions import Fraction
+    mu = float(Fraction(2.0 * ref[2][0]) * Fraction(ceq[2]) + Fraction(ref[2][1]))
     assert mu == pytest.approx(31345.33897721396, rel=1e-15)
+    # the law rejects what the reference rejects, with its text
+    with pytest.raises(C.ConfigError, match="phase-diagram extrapolated carbon outside"):
+        C.at_phase_diagram_undercooling(ref, ceq, slopes, 1e9)
 
 
 def example_inputs():
@@ -110,3 +116,44 @@ def test_parser_rejects_unknown_and_duplicate_keys(tmp_path):
         f.write("nx = 64\n")
     with pytest.raises(C.ConfigError, match="empty/duplicate key: nx"):
         C.read_config(p)
+
+
+def test_reader_reports_derived_settings(tmp_path):
+    """initial_mu from gamma_carbon is the austenite parabola's fma(2A, c, B);
+    defaults the reference computes (dy = dx, dt_min = dt / 2^20,
+    shift_gamma_carbon) come back exactly; the physics signature is the
+    one the drop-in program stores in its frames."""
+    from fractions import Fraction
+    C = cfgmod()
+    I = inputs_mod()
+    p = str(tmp_path / "g.in")
+    I.colony_input(p, 64, 32, 2, 8, 1e-9)
+    with open(p) as f:
+        lines = [l for l in f if l.split("=")[0].strip() not in ("dy", "dt_min", "shift_gamma_carbon")]
+    with open(p, "w") as f:
+        f.writelines(lines)
+    text = "".join(lines)
+    cfg = C.read_config(p)
+    assert cfg.dy == cfg.dx and cfg.dt_min == cfg.dt / 1048576.0
+    A, B = cfg.thermo[2][0], cfg.thermo[2][1]
+    want = cfg.initial_carbon if cfg.composition_mode == "gamma" else (cfg.initial_mu - B) / (2.0 * A)
+    assert cfg.shift_gamma_carbon == want
+    if cfg.composition_mode == "gamma":
+        A, B = cfg.thermo[2][0], cfg.thermo[2][1]
+        assert cfg.initial_mu == float(Fraction(2.0 * A) * Fraction(cfg.initial_carbon) + Fraction(B))
+    assert cfg.physics_signature.startswith("pearlite-v0.4-reference;")
+    assert repr(cfg.nx) in cfg.physics_signature
+    assert "nx" in text
+
+
+def test_reader_errors_are_the_reference_texts(tmp_path):
+    C = cfgmod()
+    I = inputs_mod()
+    p = str(tmp_path / "e.in")
+    I.colony_input(p, 64, 32, 2, 8, 1e-9)
+    with open(p, "a") as f:
+        f.write("FILLCUBE = {1, 2}\n")
+    with pytest.raises(C.ConfigError, match="FILLCUBE: expected 7 parameters"):
+        C.read_config(p)
+    with pytest.raises(C.ConfigError, match="cannot open input"):
+        C.read_config(str(tmp_path / "absent.in"))
