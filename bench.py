@@ -745,9 +745,18 @@ def main():
     log(f"workload {nx}x{ny}: {args.pairs} pairs, colony {strip_rows} rows, {args.undercooling} K")
     precision = {"fp64": pkg.FP64, "mixed": pkg.MIXED, "stencil32": pkg.FP32_STENCIL}[args.precision]
     s = pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=args.cells_per_thread, precision=precision)
-    s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, 0.0, 0.0, 0.0, 0, 0, 0])
-    initial_carbon = s.total_carbon()
-    s.set_state(phi, c, mu, [0.0, cfg.dt, 0.0, initial_carbon, 0.0, 0.0, 0, 0, 0])
+    # The initial state is formed on the device: the colony's separable
+    # profiles (initialize's lamella branch) and the carbon at the common mu
+    # as initialize forms it; the host copies feed the other legs.
+    initial = importlib.import_module("pearl_amd.initial")
+    th, prod = initial.colony_profiles(nx, ny, cfg.dx, cfg.dy, cfg.eps, initial.plates_from_fills(cfg.filling),
+                                       strip_rows)
+    s.init_profile(pkg.PROFILE_LAMELLA, th, prod)
+    _, initial_carbon, valid = s.init_composition(pkg.COMPOSITION_MU, cfg.initial_mu)
+    if not valid:
+        raise RuntimeError("initial state out of model range")
+    s.set_scalars([0.0, cfg.dt, 0.0, initial_carbon, 0.0, 0.0, 0, 0, 0])
+    phi, c, mu, _ = s.get_state()
 
     steps, shifts = host_loop(s, cfg, args)
 

@@ -117,6 +117,7 @@ class Fill(C.Structure):
 FILL_METHODS = ("FILLCUBE", "FILLCYLINDER", "FILLCENTERBOX2D", "FILLYJUNCTION2D", "FILLYJUNCTIONLAMELLAE2D",
                 "FILLADHEREDPEARLITEARM2D", "FILLPEARLITEELLIPSE2D")
 COMPOSITION_MU, COMPOSITION_MEAN = 0, 1
+PROFILE_LAMELLA, PROFILE_FLAT, PROFILE_CELLS = 0, 1, 2
 
 
 def fill_ops(cfg):
@@ -142,7 +143,7 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
            "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance",
            "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition", "pearl_stream_probe",
-           "pearl_diagnostics_noalloc"]
+           "pearl_diagnostics_noalloc", "pearl_init_profile"]
 
 
 def lib():
@@ -169,6 +170,7 @@ def lib():
                                                 C.POINTER(Diag), C.c_char_p, C.c_int]
         L.pearl_smooth.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(SmoothReport)]
         L.pearl_fill_phases.argtypes = [vp, C.POINTER(Fill), C.c_int]
+        L.pearl_init_profile.argtypes = [vp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int]
         L.pearl_stream_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
         L.pearl_init_composition.argtypes = [vp, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
         L.pearl_step_fixed.argtypes = [vp, C.c_double]
@@ -393,6 +395,25 @@ class Solver:
         initialize, pearl_serial.cpp:1764-1795); `ops` from fill_ops."""
         n = len(ops) if count is None else int(count)
         self._check(lib().pearl_fill_phases(self._h, ops, n))
+
+    def init_profile(self, kind: int, col=None, row=None, cell=None, phase_a: int = 0, phase_b: int = 2):
+        """Diffuse-profile phases on the device (initialize's lamella, flat
+        and circle branches, pearl_serial.cpp:1767-1771, :1787-1791) from
+        profile values: PROFILE_LAMELLA col[nx] and row[ny], PROFILE_FLAT
+        row[ny], PROFILE_CELLS cell[ny, nx]."""
+        def arr(a, size):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.size != size:
+                raise ValueError(f"profile array of {a.size} values, expected {size}")
+            return a
+        col = arr(col, self.nx)
+        row = arr(row, self.ny)
+        cell = arr(cell, self.nx * self.ny)
+        self._check(lib().pearl_init_profile(self._h, int(kind), _ptr(col) if col is not None else None,
+                                             _ptr(row) if row is not None else None,
+                                             _ptr(cell) if cell is not None else None, int(phase_a), int(phase_b)))
 
     def init_composition(self, mode: int, value: float):
         """Uniform mu and the carbon of the current phases (initialize

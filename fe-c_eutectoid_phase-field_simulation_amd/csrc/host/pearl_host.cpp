@@ -263,13 +263,14 @@ double profile(double d, double eps) {
     return d <= -half ? 1.0 : (d >= half ? 0.0 : 0.5 * (1 - std::sin(4 * d / (pi * eps))));
 }
 
-// Phases of the diffuse initializations (:1766-1792).  lamella and flat are
-// separable: a profile per column (the cementite slab of theta_columns
+// Phases of the diffuse initializations (:1766-1792) on the device.  The
+// profile values take the C library's sine, so they are formed here and the
+// device builds the phases from them (pearl_init_profile).  lamella and flat
+// are separable: a profile per column (the cementite slab of theta_columns
 // cells, periodic in x) and per row (the colony below seed_height), so each
 // sine is evaluated once per column or row; circle needs one per cell.
-void diffuse_phases(const Settings& c, HostState& st) {
+void diffuse_phases(const Settings& c, const Device& dev) {
     const int nx = c.nx, ny = c.ny;
-    const std::size_t n = st.cells();
     auto centre_x = [&](int x) { return (x + .5) * c.dx; };
     auto centre_y = [&](int y) { return (y + .5) * c.dy; };
     std::vector<double> rows(ny);
@@ -281,27 +282,19 @@ void diffuse_phases(const Settings& c, HostState& st) {
             const double xx = centre_x(x);
             cols[x] = profile(xx < width ? -std::min(xx, width - xx) : std::min(xx - width, length - xx), c.eps);
         }
+        dev.check(pearl_init_profile(dev.get(), PEARL_PROFILE_LAMELLA, cols.data(), rows.data(), nullptr, 0, 2));
+    } else if (c.initialization == "flat") {
+        dev.check(pearl_init_profile(dev.get(), PEARL_PROFILE_FLAT, nullptr, rows.data(), nullptr, c.phase_a,
+                                     c.phase_b));
+    } else {
+        std::vector<double> cells(static_cast<std::size_t>(nx) * ny);
         for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
-                const std::size_t i = static_cast<std::size_t>(y) * nx + x;
-                st.phi[i] = (1 - cols[x]) * rows[y];
-                st.phi[n + i] = cols[x] * rows[y];
-                st.phi[2 * n + i] = 1 - rows[y];
-            }
-        return;
+            for (int x = 0; x < nx; ++x)
+                cells[static_cast<std::size_t>(y) * nx + x] =
+                    profile(std::hypot(centre_x(x) - .5 * nx * c.dx, centre_y(y) - .5 * ny * c.dy) - c.radius, c.eps);
+        dev.check(pearl_init_profile(dev.get(), PEARL_PROFILE_CELLS, nullptr, nullptr, cells.data(), c.phase_a,
+                                     c.phase_b));
     }
-    const bool flat = c.initialization == "flat";
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-            const std::size_t i = static_cast<std::size_t>(y) * nx + x;
-            const double u =
-                flat ? rows[y]
-                     : profile(std::hypot(centre_x(x) - .5 * nx * c.dx, centre_y(y) - .5 * ny * c.dy) - c.radius, c.eps);
-            double p[3] = {0, 0, 0};
-            p[c.phase_a] = u;
-            p[c.phase_b] = 1 - u;
-            for (int a = 0; a < 3; ++a) st.phi[a * n + i] = p[a];
-        }
 }
 
 // ---------------------------------------------------------------------------
@@ -373,8 +366,8 @@ private:
             rep.after = phase_fractions(state_);
             write_phases(out_ / "init_smoothed.h5", state_, cfg_, "init_smoothed", rep.steps);
         } else {
-            diffuse_phases(cfg_, state_);
-            init.check(pearl_set_state(init.get(), state_.phi.data(), state_.carbon.data(), state_.mu.data(), &zero));
+            diffuse_phases(cfg_, init);
+            init.download_phases(state_);
             rep.before = rep.after = phase_fractions(state_);
             smooth(init, rep, 0, 0);  // the interface energy alone
             rep.converged = true;

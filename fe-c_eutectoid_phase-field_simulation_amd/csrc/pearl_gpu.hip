@@ -1519,6 +1519,57 @@ int pearl_fill_phases(pearl_ctx* c, const pearl_fill* ops, int count) {
     }
 }
 
+int pearl_init_profile(pearl_ctx* c, int kind, const double* col, const double* row, const double* cell,
+                       int phase_a, int phase_b) {
+    try {
+        int st = ctx_check(c, 0);
+        if (st) return st;
+        const KP& P = c->kp[0];
+        const size_t nx = static_cast<size_t>(P.nx), ny = static_cast<size_t>(P.ny);
+        bool ok = kind >= PEARL_PROFILE_LAMELLA && kind <= PEARL_PROFILE_CELLS;
+        if (ok && kind != PEARL_PROFILE_LAMELLA)
+            ok = phase_a >= 0 && phase_a <= 2 && phase_b >= 0 && phase_b <= 2 && phase_a != phase_b;
+        // the arrays a kind reads: (pointer, length)
+        const double* arr[2] = {nullptr, nullptr};
+        size_t len[2] = {0, 0};
+        if (kind == PEARL_PROFILE_LAMELLA) arr[0] = col, len[0] = nx, arr[1] = row, len[1] = ny;
+        else if (kind == PEARL_PROFILE_FLAT) arr[0] = row, len[0] = ny;
+        else arr[0] = cell, len[0] = nx * ny;
+        for (int k = 0; ok && k < 2; ++k) {
+            if (!len[k]) continue;
+            if (!arr[k]) ok = false;
+            for (size_t j = 0; ok && j < len[k]; ++j) ok = std::isfinite(arr[k][j]);
+        }
+        if (!ok) return fail(c, PEARL_ERR_ARGUMENT, "invalid profile");
+        st = read_scalars(c);
+        if (st) return st;
+        double* d = nullptr;
+        const size_t total = len[0] + len[1];
+        HIP_CHECK(c, hipMalloc(&d, total * sizeof(double)));
+        hipError_t e = hipMemcpyAsync(d, arr[0], len[0] * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && len[1])
+            e = hipMemcpyAsync(d + len[0], arr[1], len[1] * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            const double* d_col = kind == PEARL_PROFILE_LAMELLA ? d : nullptr;
+            const double* d_row = kind == PEARL_PROFILE_LAMELLA ? d + len[0] : (kind == PEARL_PROFILE_FLAT ? d : nullptr);
+            const double* d_cell = kind == PEARL_PROFILE_CELLS ? d : nullptr;
+            PEARL_LAUNCH(c, k_profile, dim3(static_cast<unsigned>(P.nb)), c->B, 0, kind, d_col, d_row, d_cell, phase_a,
+                         phase_b);
+            e = hipGetLastError();
+        }
+        const hipError_t e2 = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = e2;
+        (void)hipFree(d);
+        if (e != hipSuccess) return hip_fail(c, e, "k_profile");
+        DevScal& h = c->hS[0];
+        h.e0_valid = 0;
+        h.m0_valid = 0;
+        return write_scalars(c);
+    } catch (const std::bad_alloc&) {
+        return alloc_fail(c);
+    }
+}
+
 int pearl_init_composition(pearl_ctx* c, int mode, double value, double* mu_out, double* carbon_out,
                            int* valid_out) {
     try {

@@ -2202,6 +2202,35 @@ __global__ __launch_bounds__(TPB) void k_fill(Bufs B, int d, const FillOp* __res
     });
 }
 
+// Diffuse-profile phases of the current state (initialize's lamella, flat
+// and circle branches, :1767-1771, :1787-1791) from host profile values:
+// kind 0 col[x], row[y]; kind 1 row[y]; kind 2 cell[i].
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_profile(Bufs B, int d, int kind, const double* __restrict__ col,
+                                                 const double* __restrict__ row, const double* __restrict__ cell,
+                                                 int pa, int pb) {
+    const KP P = B.kp[d];
+    const int lb = blockIdx.x;
+    double* __restrict__ phi = B.phi[B.S[d].cur] + 3 * P.off;
+    const int n = P.n;
+    const Col c = column<TY>(P, lb, P.ny);
+    for_column<TY>(P, c, [&](const int i, const int y) {
+        double p[3] = {0.0, 0.0, 0.0};
+        if (kind == 0) {
+            const double th = col[c.x], product = row[y];
+            p[0] = (1 - th) * product;
+            p[1] = th * product;
+            p[2] = 1 - product;
+        } else {
+            const double u = kind == 1 ? row[y] : cell[i];
+            p[pa] = u;
+            p[pb] = 1 - u;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) phi[a * n + i] = p[a];
+    });
+}
+
 // Sums over the cells of the carbon intercept and the susceptibility of
 // the current phases (the mean_carbon branch of initialize :1814-1822),
 // blocked order; q2 flags a cell whose mixture the reference rejects.

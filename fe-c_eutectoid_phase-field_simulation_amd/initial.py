@@ -45,9 +45,11 @@ def profile(distance: np.ndarray, eps: float) -> np.ndarray:
     return out
 
 
-def diffuse_colony(nx: int, ny: int, dx: float, dy: float, eps: float, plates, strip_rows: int) -> np.ndarray:
-    """phi[3, ny, nx] for cementite plates [(x_start, width)] in cells, the
-    colony occupying rows [0, strip_rows).  x is periodic."""
+def colony_profiles(nx: int, ny: int, dx: float, dy: float, eps: float, plates, strip_rows: int):
+    """The separable profiles of diffuse_colony: th[nx], the cementite
+    profile of each column's signed distance to the nearest plate, and
+    prod[ny], the colony profile of each row (the PROFILE_LAMELLA tables of
+    Solver.init_profile)."""
     L = nx * dx
     xx = (np.arange(nx) + 0.5) * dx
     # signed distance to the nearest cementite plate (negative inside)
@@ -59,9 +61,15 @@ def diffuse_colony(nx: int, ny: int, dx: float, dy: float, eps: float, plates, s
             d = np.where(inside, -np.minimum(xx - a - shift, b + shift - xx),
                          np.minimum(np.abs(xx - a - shift), np.abs(xx - b - shift)))
             dist = np.where(np.abs(d) < np.abs(dist), d, dist)
-    th = profile(dist, eps)[None, :]
     yy = (np.arange(ny) + 0.5) * dy
-    prod = profile(yy - strip_rows * dy, eps)[:, None]
+    return profile(dist, eps), profile(yy - strip_rows * dy, eps)
+
+
+def diffuse_colony(nx: int, ny: int, dx: float, dy: float, eps: float, plates, strip_rows: int) -> np.ndarray:
+    """phi[3, ny, nx] for cementite plates [(x_start, width)] in cells, the
+    colony occupying rows [0, strip_rows).  x is periodic."""
+    th, prod = colony_profiles(nx, ny, dx, dy, eps, plates, strip_rows)
+    th, prod = th[None, :], prod[:, None]
     phi = np.empty((3, ny, nx))
     phi[0] = (1 - th) * prod
     phi[1] = th * prod

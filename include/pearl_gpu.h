@@ -251,6 +251,29 @@ int pearl_smooth(pearl_ctx* ctx, int max_steps, int min_steps, double energy_rto
  * parameter, or a pair count below 1). */
 int pearl_fill_phases(pearl_ctx* ctx, const pearl_fill* ops, int count);
 
+/* Kinds of pearl_init_profile. */
+#define PEARL_PROFILE_LAMELLA 0 /* initialization = lamella :1767-1771 */
+#define PEARL_PROFILE_FLAT 1    /* initialization = flat    :1787-1791 */
+#define PEARL_PROFILE_CELLS 2   /* one value per cell: initialization = circle :1787-1791 */
+
+/* The diffuse-profile initializations on the device (replaces the lamella,
+ * flat and circle branches of initialize's loop, :1764-1795).  The profile
+ * values come from the caller, because profile (:563-569) takes the C
+ * library's sine; the device forms the three phases of every cell from them
+ * with the reference's expressions:
+ *   LAMELLA: col[nx] = profile of column x's signed distance to the theta
+ *            band, row[ny] = profile(y - seed_height):
+ *            phi = ((1 - col) row, col row, 1 - row)
+ *   FLAT:    row[ny]: phi[phase_a] = row, phi[phase_b] = 1 - row, else 0
+ *   CELLS:   cell[ny*nx] (row-major): phi[phase_a] = cell, phi[phase_b] =
+ *            1 - cell, else 0
+ * Arrays a kind does not use may be NULL.  Carbon and mu are left as they
+ * are (pearl_init_composition follows).  Errors: PEARL_ERR_ARGUMENT
+ * ("invalid profile": unknown kind, a missing array, a phase pair outside
+ * 0..2 or equal, a non-finite value). */
+int pearl_init_profile(pearl_ctx* ctx, int kind, const double* col, const double* row, const double* cell,
+                       int phase_a, int phase_b);
+
 /* Carbon and chemical potential of the current phases (replaces the end of
  * initialize, :1813-1836).  mode PEARL_COMPOSITION_MU: mu = value.  mode
  * PEARL_COMPOSITION_MEAN: value is the mean carbon and

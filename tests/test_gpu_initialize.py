@@ -20,7 +20,13 @@ the C ABI, against the reference's own functions (oracle/_ref).
 4. The whole device initialization (labels, pearl_smooth, carbon) against
    the reference's initialize: equal phases, carbon and mu where every
    smoothing decision agrees, which the equal step counts and energies show.
-5. Argument and mixture errors."""
+5. Argument and mixture errors.
+6. The diffuse-profile initializations (pearl_init_profile): phases
+   bitwise the reference's expressions of its lamella, flat and circle
+   branches (:1767-1771, :1787-1791) from given profile values, at two tile
+   heights and a width off the tile multiple; argument errors.  (The whole
+   diffuse initialization of the drop-in program against the reference
+   program's is tests/test_gpu_host_program.py.)"""
 import importlib
 import math
 import os
@@ -221,3 +227,39 @@ def test_composition_reports_invalid_mixture(pkg, tmp_path, mode):
         with pytest.raises(pkg.PearlError) as e:
             s.init_composition(mode, cfg.initial_mu if mode == 0 else 0.03)
     assert e.value.code == pkg.ERR_MIXTURE and "Invalid mixture coefficients" in str(e.value)
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_init_profile_forms_the_reference_phases(pkg, tmp_path, k):
+    nx, ny = 300, 170
+    path = recipe_input(tmp_path, nx, ny, 0, "FILLCENTERBOX2D")
+    cfg = mod("config").read_config(path)
+    rng = np.random.default_rng(11)
+    col, row, cell = rng.random(nx), rng.random(ny), rng.random((ny, nx))
+    col[:7], row[:5] = 1.0, 0.0  # the profile's flat parts
+    c0, mu0 = np.full((ny, nx), 0.03), np.full((ny, nx), cfg.initial_mu)
+    with pkg.Solver(cfg.solver_params(), cfg.initial_mu, cells_per_thread=k) as s:
+        s.set_state(np.zeros((3, ny, nx)), c0, mu0, scalars(cfg))
+        s.init_profile(pkg.PROFILE_LAMELLA, col, row)
+        phi, c, mu, _ = s.get_state()
+        th, pr = col[None, :], row[:, None]
+        want = np.stack([(1 - th) * pr, th * pr, np.broadcast_to(1 - pr, (ny, nx))])
+        assert np.array_equal(phi, want)
+        assert np.array_equal(c, c0) and np.array_equal(mu, mu0)
+        s.init_profile(pkg.PROFILE_FLAT, row=row, phase_a=2, phase_b=0)
+        phi = s.get_state()[0]
+        assert np.array_equal(phi[2], np.broadcast_to(row[:, None], (ny, nx)))
+        assert np.array_equal(phi[0], np.broadcast_to(1 - row[:, None], (ny, nx)))
+        assert not phi[1].any()
+        s.init_profile(pkg.PROFILE_CELLS, cell=cell, phase_a=1, phase_b=2)
+        phi = s.get_state()[0]
+        assert np.array_equal(phi[1], cell) and np.array_equal(phi[2], 1 - cell) and not phi[0].any()
+        for kind, kw in ((3, dict(row=row)), (pkg.PROFILE_LAMELLA, dict(col=col)),
+                         (pkg.PROFILE_FLAT, dict(row=row, phase_a=1, phase_b=1)),
+                         (pkg.PROFILE_CELLS, dict(cell=cell, phase_a=0, phase_b=3)),
+                         (pkg.PROFILE_FLAT, dict(row=np.where(np.arange(ny) == 9, np.nan, row)))):
+            with pytest.raises(pkg.PearlError) as e:
+                s.init_profile(kind, **kw)
+            assert e.value.code == pkg.ERR_ARGUMENT and "invalid profile" in str(e.value)
+        # the phases of the last valid call are still in place
+        assert np.array_equal(s.get_state()[0][1], cell)
