@@ -26,7 +26,11 @@ Prints one JSON line:
             is one, and traffic_GBps that traffic over the live launch time;
             copy_ceiling_GBps is a hand-written 1 GiB device copy measured in
             the same run (pearl_stream_probe), frac_of_copy the dominant
-            kernel's rate against it
+            kernel's rate against it; for every cell kernel,
+            bare_pattern_GBps is a bare kernel with the same read and write
+            planes in the same geometry (pearl_stream_mix) and
+            bare_pattern_over_kernel_time its launch time over the kernel's
+            (near 1: the kernel runs at its access pattern's rate)
   cpu_baseline  the reference solver compiled from source (or the oracle
             restatement when that is absent) on a bounded sample of the same
             workload on this host's cores
@@ -811,10 +815,16 @@ def main():
                     f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s")
     # Measured bandwidth ceiling beside the vendor peak: a hand-written plain
     # copy of 1 GiB (pearl_stream_probe; read + write counted).
-    copy_gbs = None
+    copy_gbs, mix = None, {}
     if not args.no_profile:
         copy_gbs = pkg.stream_probe(0, 1 << 30, 20)
         log(f"copy ceiling: {copy_gbs:.1f} GB/s (1 GiB device copy, pearl_stream_probe)")
+        if args.precision == "fp64":
+            # each kernel's own access pattern without its arithmetic
+            for name, (r, w) in pkg.KERNEL_STREAMS.items():
+                mix[name] = pkg.stream_mix(r, w, nx, ny, s.layout()[1])
+                log(f"  bare pattern {name:16s} {r} read + {w} written planes: {mix[name][0]:7.1f} us, "
+                    f"{mix[name][1]:7.1f} GB/s")
     mcs = nx * ny * args.steps / wall / 1e6
     rows_per_tile = s.layout()[1]
     roof = None
@@ -832,6 +842,13 @@ def main():
                 "kernels": {k: {"achieved_GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
                                 "avg_launch_ms": round(v["ms"] / v["launches"], 5), "launches": v["launches"]}
                             for k, v in active.items() if v["bytes"] > 0}}
+        for k, v in roof["kernels"].items():
+            if k in mix:
+                # a bare kernel with this kernel's planes, no arithmetic: the
+                # rate its access pattern reaches (a kernel with better issue,
+                # e.g. two threads per column, can beat it)
+                v["bare_pattern_GBps"] = round(mix[k][1], 1)
+                v["bare_pattern_over_kernel_time"] = round(mix[k][0] / (v["avg_launch_ms"] * 1e3), 4)
         measured = os.path.join(PROFILES, f"hbm_traffic_{args.precision}_{nx}x{ny}_k{rows_per_tile}.json")
         if os.path.exists(measured):
             with open(measured) as fh:

@@ -143,7 +143,7 @@ EXPORTS = ["pearl_create", "pearl_destroy", "pearl_last_error", "pearl_set_state
            "pearl_batch_create", "pearl_batch_size", "pearl_batch_set_state", "pearl_batch_get_state",
            "pearl_batch_run", "pearl_smooth", "pearl_step_fixed", "pearl_adjoint", "pearl_set_stop_tolerance",
            "pearl_run_fixed", "pearl_fill_phases", "pearl_init_composition", "pearl_stream_probe",
-           "pearl_diagnostics_noalloc", "pearl_init_profile"]
+           "pearl_diagnostics_noalloc", "pearl_init_profile", "pearl_stream_mix"]
 
 
 def lib():
@@ -172,6 +172,7 @@ def lib():
         L.pearl_fill_phases.argtypes = [vp, C.POINTER(Fill), C.c_int]
         L.pearl_init_profile.argtypes = [vp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_int]
         L.pearl_stream_probe.argtypes = [C.c_int, C.c_longlong, C.c_int, _dp]
+        L.pearl_stream_mix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.pearl_init_composition.argtypes = [vp, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
         L.pearl_step_fixed.argtypes = [vp, C.c_double]
         L.pearl_set_stop_tolerance.argtypes = [vp, C.c_double]
@@ -217,6 +218,25 @@ def stream_probe(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> floa
     if rc != OK:
         raise PearlError(rc, "pearl_stream_probe failed")
     return g.value
+
+
+# FP64 planes each cell kernel reads and writes once per cell (the access
+# pattern of its algorithmic bytes; the FP64 plain form).
+KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 7), "cg_apply": (4, 2), "cg_update": (5, 3),
+                  "true_residual": (5, 2), "finalize_accept": (6, 2)}
+
+
+def stream_mix(reads: int, writes: int, nx: int, ny: int, rows_per_thread: int = 4, reps: int = 20,
+               device: int = 0):
+    """(us per launch, GB/s) of a bare kernel with `reads` and `writes` FP64
+    planes in the cell kernels' geometry (pearl_stream_mix): the rate that
+    access pattern reaches without arithmetic."""
+    us, g = C.c_double(0), C.c_double(0)
+    rc = lib().pearl_stream_mix(int(device), int(reads), int(writes), int(nx), int(ny), int(rows_per_thread),
+                                int(reps), C.byref(us), C.byref(g))
+    if rc != OK:
+        raise PearlError(rc, "pearl_stream_mix failed")
+    return us.value, g.value
 
 
 def diagnostics_noalloc(params, mu_reference: float, phi_ptr: int, carbon_ptr: int, mu_ptr: int,

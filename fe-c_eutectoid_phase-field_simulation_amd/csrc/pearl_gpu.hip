@@ -1743,6 +1743,99 @@ int pearl_stream_probe(int device, long long bytes, int reps, double* gbps) {
 
 }  // extern "C"
 
+// The streams of a cell kernel without its arithmetic: `reads` planes read
+// and `writes` planes written, nontemporal, one column of TY cells per
+// thread over 256-column tiles, the planes spaced as in the arena (the cell
+// kernels' geometry).  At most 8 of
+// each; the counts are uniform, so their tests are scalar branches.
+constexpr int kMixMax = 8;
+template <int TY>
+__global__ __launch_bounds__(TPB) void k_stream_mix(const double* const* __restrict__ in, double* const* __restrict__ out,
+                                                  int reads, int writes, int nx, int ny, int tiles_x) {
+    const int lb = static_cast<int>(blockIdx.x), tx = lb % tiles_x, ty = lb / tiles_x;
+    const int x = tx * TPB + static_cast<int>(threadIdx.x), y0 = ty * TY;
+    if (x >= nx) return;
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        if (y0 + r >= ny) break;
+        const long long i = static_cast<long long>(y0 + r) * nx + x;
+        double v = 0;
+#pragma unroll
+        for (int q = 0; q < kMixMax; ++q)
+            if (q < reads) v += __builtin_nontemporal_load(in[q] + i);
+#pragma unroll
+        for (int q = 0; q < kMixMax; ++q)
+            if (q < writes) __builtin_nontemporal_store(v + q, out[q] + i);
+    }
+}
+
+extern "C" int pearl_stream_mix(int device, int reads, int writes, int nx, int ny, int rows_per_thread, int reps,
+                                double* us, double* gbps) {
+    const bool rows_ok = rows_per_thread == 1 || rows_per_thread == 2 || rows_per_thread == 4 ||
+                         rows_per_thread == 8 || rows_per_thread == 16;
+    if (!us || !gbps || reads < 1 || reads > kMixMax || writes < 1 || writes > kMixMax || nx < 1 || ny < 1 ||
+        !rows_ok || reps < 1 || static_cast<long long>(nx) * ny > (1LL << 31))
+        return PEARL_ERR_ARGUMENT;
+    *us = *gbps = 0;
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return PEARL_ERR_DEVICE;
+    }
+    // one allocation with the planes spaced as in a context's arena (the
+    // cells plus 576)
+    const size_t n = static_cast<size_t>(nx) * static_cast<size_t>(ny), spacing = n + 576;
+    std::vector<double*> planes(static_cast<size_t>(reads + writes), nullptr);
+    double* mem = nullptr;
+    double** d_ptr = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&mem, planes.size() * spacing * sizeof(double));
+    for (size_t k = 0; k < planes.size() && e == hipSuccess; ++k) planes[k] = mem + k * spacing;
+    if (e == hipSuccess) e = hipMalloc(&d_ptr, planes.size() * sizeof(double*));
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int q = 0; q < reads && e == hipSuccess; ++q) e = hipMemsetAsync(planes[q], 0, n * sizeof(double), st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_ptr, planes.data(), planes.size() * sizeof(double*), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const int tiles_x = (nx + TPB - 1) / TPB;
+        const unsigned blocks = static_cast<unsigned>(tiles_x * ((ny + rows_per_thread - 1) / rows_per_thread));
+        const double* const* in = d_ptr;
+        double* const* out = d_ptr + reads;
+        auto launch = [&] {
+            switch (rows_per_thread) {
+                case 1: hipLaunchKernelGGL(k_stream_mix<1>, dim3(blocks), dim3(TPB), 0, st, in, out, reads, writes, nx, ny, tiles_x); break;
+                case 2: hipLaunchKernelGGL(k_stream_mix<2>, dim3(blocks), dim3(TPB), 0, st, in, out, reads, writes, nx, ny, tiles_x); break;
+                case 4: hipLaunchKernelGGL(k_stream_mix<4>, dim3(blocks), dim3(TPB), 0, st, in, out, reads, writes, nx, ny, tiles_x); break;
+                case 8: hipLaunchKernelGGL(k_stream_mix<8>, dim3(blocks), dim3(TPB), 0, st, in, out, reads, writes, nx, ny, tiles_x); break;
+                default: hipLaunchKernelGGL(k_stream_mix<16>, dim3(blocks), dim3(TPB), 0, st, in, out, reads, writes, nx, ny, tiles_x); break;
+            }
+        };
+        for (int r = 0; r < 3; ++r) launch();
+        e = hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms > 0) {
+            *us = 1e3 * ms / reps;
+            *gbps = static_cast<double>(reads + writes) * sizeof(double) * static_cast<double>(n) / (*us * 1e3);
+        }
+    }
+    if (e1) (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (st) (void)hipStreamDestroy(st);
+    if (d_ptr) (void)hipFree(d_ptr);
+    if (mem) (void)hipFree(mem);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e == hipSuccess ? PEARL_OK : (e == hipErrorOutOfMemory ? PEARL_ERR_ALLOC : PEARL_ERR_DEVICE);
+}
+
 // ---------------------------------------------------------------------------
 // pearl_diagnostics_noalloc: the snapshot reductions of a state held in the
 // caller's device memory, for callers that cannot allocate device memory.

@@ -386,6 +386,18 @@ int pearl_counters(const pearl_ctx* ctx, long long* attempts, long long* rejecte
  * writes `bytes`).  The measured ceiling reported beside the vendor peak. */
 int pearl_stream_probe(int device, long long bytes, int reps, double* gbps);
 
+/* Measurement utility, no reference counterpart: the rate a cell kernel's
+ * access pattern can reach without its arithmetic.  A kernel reads `reads`
+ * and writes `writes` FP64 planes of nx*ny cells (1..8 each), nontemporal,
+ * one column of rows_per_thread (1, 2, 4, 8, 16) cells per thread over
+ * 256-column tiles, in one allocation with the planes spaced as in a
+ * context's arena, as the cell kernels do.  `reps` timed launches after
+ * three warm-up launches, HIP events on a stream of its own: *us per launch
+ * and *gbps = (reads + writes) * 8 * nx * ny / seconds / 1e9.  Allocates
+ * the planes for the call. */
+int pearl_stream_mix(int device, int reads, int writes, int nx, int ny, int rows_per_thread, int reps, double* us,
+                     double* gbps);
+
 #ifdef __cplusplus
 }
 #endif

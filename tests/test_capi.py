@@ -93,6 +93,15 @@ def test_argument_checks_before_any_device_call():
     for nbytes, reps in ((15, 1), (0, 1), (-16, 1), (1 << 20, 0), ((1 << 35) + 16, 1)):
         assert L.pearl_stream_probe(0, nbytes, reps, ctypes.byref(g)) == pkg.ERR_ARGUMENT, (nbytes, reps)
     assert L.pearl_stream_probe(0, 1 << 20, 1, None) == pkg.ERR_ARGUMENT
+    us = ctypes.c_double(1.0)
+    for reads, writes, nx, ny, rows, reps in ((0, 1, 64, 8, 4, 1), (9, 1, 64, 8, 4, 1), (1, 0, 64, 8, 4, 1),
+                                              (1, 9, 64, 8, 4, 1), (1, 1, 0, 8, 4, 1), (1, 1, 64, 8, 3, 1),
+                                              (1, 1, 64, 8, 4, 0), (1, 1, 1 << 16, 1 << 16, 4, 1)):
+        assert L.pearl_stream_mix(0, reads, writes, nx, ny, rows, reps, ctypes.byref(us),
+                                  ctypes.byref(g)) == pkg.ERR_ARGUMENT
+    assert L.pearl_stream_mix(0, 1, 1, 64, 8, 4, 1, None, ctypes.byref(g)) == pkg.ERR_ARGUMENT
+    # an empty profile kind or a null context is rejected before any device call
+    assert L.pearl_init_profile(None, 0, None, None, None, 0, 2) == pkg.ERR_ARGUMENT
     # a null context is an argument error for every context call
     assert L.pearl_fill_phases(None, None, 0) == pkg.ERR_ARGUMENT
     mu, carbon, valid = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
