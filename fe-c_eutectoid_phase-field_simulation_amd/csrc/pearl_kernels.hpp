@@ -879,42 +879,62 @@ __device__ __forceinline__ void propose_z32(const KP& P, int y, const double (&p
 
 // propose_phi :961-1002 for one cell: p and the south/north phases from the
 // column registers, west/east loaded.  v[0] = max change, v[1] = projection
-// failure.  F32: the FP32-stencil form (propose_z32); the carried north face
-// terms are then FP32.
-template <bool F32, class W>
-__device__ __forceinline__ void propose_cell(const KP& P, const double* __restrict__ phi, int n, int i, int y,
-                                             const Col& col, const double (&ps)[3], const double (&p)[3],
-                                             const double (&pn)[3], double m, double dt, double* __restrict__ out,
-                                             double (&v)[2], const W* south, W (&north)[3]) {
+// failure.  The FP32-stencil form (propose_z32); the carried north face
+// terms are FP32.
+__device__ __forceinline__ void propose_cell32(const KP& P, const double* __restrict__ phi, int n, int i, int y,
+                                               const Col& col, const double (&ps)[3], const double (&p)[3],
+                                               const double (&pn)[3], double m, double dt, double* __restrict__ out,
+                                               double (&v)[2], const float* south, float (&north)[3]) {
     double qw[3], qe[3];
     load_phi(phi, n, i + col.dw, qw);
     load_phi(phi, n, i + col.de, qe);
     double z[3], zp[3];
-    bool finite, projected = true;
-    if constexpr (F32) {
-        propose_z32(P, y, p, qw, qe, ps, pn, m, dt, z, south, north);
-        finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
-        if (finite) projected = project_simplex(z, P.r3, zp);
-    } else {
-        // the cell's constant divisions as one checked run
-        checked_run([&](auto& div) {
-            double force[3], mean, scale;
-            propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north, div);
-            finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
-            projected = !finite || project_simplex(z, P.r3, zp, div);
-        });
+    propose_z32(P, y, p, qw, qe, ps, pn, m, dt, z, south, north);
+    if (!(__builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]))) {
+        v[0] = __builtin_inf();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) st_once(out + a * n + i, p[a]);
+        return;
+    }
+    if (!project_simplex(z, P.r3, zp)) v[1] = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) st_once(out + a * n + i, zp[a]);
+}
+
+// The FP64 form of the same, with the constant-division policy `div`.  With
+// DivRun, a cell whose range check fails stores and accumulates nothing and
+// returns false: the kernel computes it again with DivIEEE after its march
+// (its north terms are then not carried; the next cell forms its south
+// terms itself).
+template <class Div>
+__device__ __forceinline__ bool propose_cell64(const KP& P, const double* __restrict__ phi, int n, int i, int y,
+                                               const Col& col, const double (&ps)[3], const double (&p)[3],
+                                               const double (&pn)[3], double m, double dt, double* __restrict__ out,
+                                               double (&v)[2], const double* south, double (&north)[3], Div& div) {
+    double qw[3], qe[3];
+    load_phi(phi, n, i + col.dw, qw);
+    load_phi(phi, n, i + col.de, qe);
+    double z[3], zp[3], force[3], mean, scale;
+    propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north, div);
+    const bool finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
+    const bool projected = !finite || project_simplex(z, P.r3, zp, div);
+    if constexpr (std::is_same<Div, DivRun>::value) {
+        if (__builtin_expect(!div.ok(), 0)) return false;
     }
     if (!finite) {
         v[0] = __builtin_inf();
 #pragma unroll
         for (int a = 0; a < 3; ++a) st_once(out + a * n + i, p[a]);
-        return;
+        return true;
     }
     if (!projected) v[1] = 1.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[0] = dmaxd(v[0], __builtin_fabs(zp[a] - p[a]));
 #pragma unroll
     for (int a = 0; a < 3; ++a) st_once(out + a * n + i, zp[a]);
+    return true;
 }
 
 // LDS tile of a cell kernel: TY rows of the block's 256 columns, the row
@@ -987,6 +1007,9 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     double v[2] = {0.0, 0.0};
     W carry[3] = {0, 0, 0};
     const Col col = column<TY>(P, w.lb, P.ny);
+    // FP64: rows whose constant-division range check failed, redone below
+    unsigned redo = 0;
+    bool carried = false;
     march<TY, Phases>(
         P, col,
         [&](const int j) {
@@ -997,11 +1020,35 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
         [&](const int i, const int y, const Phases& s, const Phases& c, const Phases& nn) {
             // the previous cell of the march (y - 1) computed this cell's south terms
             W north[3] = {0, 0, 0};
-            propose_cell<F32, W>(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
-                                 y > col.y0 ? carry : nullptr, north);
+            if constexpr (F32) {
+                propose_cell32(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
+                               y > col.y0 ? carry : nullptr, north);
+            } else {
+                DivRun fast;
+                carried = propose_cell64(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
+                                         carried ? carry : nullptr, north, fast);
+                if (!carried) redo |= 1u << (y - col.y0);
+            }
 #pragma unroll
             for (int q = 0; q < 3; ++q) carry[q] = north[q];
         });
+    if constexpr (!F32) {
+        // Cells where a numerator left the fast quotient's range (rare): every
+        // term again with IEEE quotients, the south terms formed directly.
+        // Out of the march, this path adds no registers to it.
+        if (__builtin_expect(redo != 0, 0)) {
+            for (int r = 0; r < TY; ++r) {
+                if (!(redo & (1u << r))) continue;
+                const int y = col.y0 + r, i = col.i0 + r * P.nx;
+                double ps[3], pc[3], pn[3], north[3];
+                load_phi(phi, n, y == 0 ? i : i - P.nx, ps);
+                load_phi(phi, n, i, pc);
+                load_phi(phi, n, y == P.ny - 1 ? i : i + P.nx, pn);
+                DivIEEE ieee;
+                propose_cell64(P, phi, n, i, y, col, ps, pc, pn, ld_once(mu + i), dt, out, v, nullptr, north, ieee);
+            }
+        }
+    }
     block_partial<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
 

@@ -50,6 +50,37 @@ def test_propose_phi_bitwise_vs_reference(pkg, name, k):
     assert np.array_equal(phit, g["propose_phi"])
 
 
+@pytest.mark.parametrize("k", [1, 2, 4, 16])
+def test_propose_phi_out_of_range_numerators(pkg, O, k):
+    """A patch of cells whose phase fractions are tiny (down to subnormal)
+    makes face and driving-force numerators leave the fast constant
+    quotient's range: those cells are computed again with IEEE quotients
+    after the march, and their north face terms are not carried.  Bitwise
+    the oracle's sequential propose_phi, which divides with IEEE quotients
+    throughout."""
+    g = load_golden("small_48x40")
+    ny, nx = g["mu0"].shape
+    phi = np.array(g["phi0"], dtype=np.float64).reshape(3, ny, nx)
+    rng = np.random.default_rng(17)
+    ys, xs = slice(5, 21), slice(3, 31)
+    vals = np.array([0.0, 5e-324, 3e-310, 1e-200, 1e-160, 2e-155, 1e-100])
+    t = rng.choice(vals, size=(2, 16, 28))
+    phi[0, ys, xs], phi[1, ys, xs] = t[0], t[1]
+    phi[2, ys, xs] = 1.0 - t[0] - t[1]
+    # and single tiny cells next to ordinary interface cells
+    for y, x in ((30, 10), (31, 10), (30, 11), (2, 40)):
+        phi[:, y, x] = (1e-170, 3e-165, 1.0)
+    say("propose_phi out-of-range numerators, k", k)
+    with pkg.Solver(g["pd"], float(g["sc0"][9]), cells_per_thread=k) as s:
+        s.set_state(phi, g["c0"], g["mu0"], g["sc0"][:9])
+        ch, got = s.propose_phi(float(g["dt_probe"]))
+    P = O.params_from_dict(g["pd"])
+    want_ch, want, st = O.propose_phi(P, phi, np.asarray(g["mu0"], dtype=np.float64), float(g["dt_probe"]))
+    assert ch == want_ch
+    assert np.array_equal(got.reshape(want.shape), want)
+    assert np.array_equal(np.signbit(got.reshape(want.shape)), np.signbit(want))
+
+
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
 def test_implicit_diffusion(pkg, O, name, k):
