@@ -904,15 +904,16 @@ __device__ __forceinline__ void propose_cell32(const KP& P, const double* __rest
 }
 
 // The FP64 form of the same, with the constant-division policy `div`.  With
-// DivRun, a cell whose range check fails stores and accumulates nothing and
-// returns false: the kernel computes it again with DivIEEE after its march
-// (its north terms are then not carried; the next cell forms its south
-// terms itself).
+// DivRun it returns whether the cell's range check held; the cell is stored
+// and accumulated only when it held and `commit` is set (commit: the south
+// terms came from a cell whose check held).  Otherwise the kernel computes
+// the cell again with DivIEEE after its march.
 template <class Div>
 __device__ __forceinline__ bool propose_cell64(const KP& P, const double* __restrict__ phi, int n, int i, int y,
                                                const Col& col, const double (&ps)[3], const double (&p)[3],
                                                const double (&pn)[3], double m, double dt, double* __restrict__ out,
-                                               double (&v)[2], const double* south, double (&north)[3], Div& div) {
+                                               double (&v)[2], const double* south, double (&north)[3], Div& div,
+                                               bool commit = true) {
     double qw[3], qe[3];
     load_phi(phi, n, i + col.dw, qw);
     load_phi(phi, n, i + col.de, qe);
@@ -921,7 +922,8 @@ __device__ __forceinline__ bool propose_cell64(const KP& P, const double* __rest
     const bool finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
     const bool projected = !finite || project_simplex(z, P.r3, zp, div);
     if constexpr (std::is_same<Div, DivRun>::value) {
-        if (__builtin_expect(!div.ok(), 0)) return false;
+        const bool held = div.ok();
+        if (__builtin_expect(!(held && commit), 0)) return held;
     }
     if (!finite) {
         v[0] = __builtin_inf();
@@ -1007,9 +1009,10 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
     double v[2] = {0.0, 0.0};
     W carry[3] = {0, 0, 0};
     const Col col = column<TY>(P, w.lb, P.ny);
-    // FP64: rows whose constant-division range check failed, redone below
+    // FP64: rows whose constant-division range check failed, or whose south
+    // terms came from such a row, redone below
     unsigned redo = 0;
-    bool carried = false;
+    bool held = true;
     march<TY, Phases>(
         P, col,
         [&](const int j) {
@@ -1025,17 +1028,19 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
                                y > col.y0 ? carry : nullptr, north);
             } else {
                 DivRun fast;
-                carried = propose_cell64(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
-                                         carried ? carry : nullptr, north, fast);
-                if (!carried) redo |= 1u << (y - col.y0);
+                const bool south_held = held;
+                held = propose_cell64(P, phi, n, i, y, col, s.p, c.p, nn.p, ld_once(mu + i), dt, out, v,
+                                      y > col.y0 ? carry : nullptr, north, fast, south_held);
+                if (!(held && south_held)) redo |= 1u << (y - col.y0);
             }
 #pragma unroll
             for (int q = 0; q < 3; ++q) carry[q] = north[q];
         });
     if constexpr (!F32) {
-        // Cells where a numerator left the fast quotient's range (rare): every
-        // term again with IEEE quotients, the south terms formed directly.
-        // Out of the march, this path adds no registers to it.
+        // Cells where a numerator left the fast quotient's range (rare), and
+        // the cell above each: every term again with IEEE quotients, the
+        // south terms formed directly.  Out of the march, this path adds few
+        // registers to it.
         if (__builtin_expect(redo != 0, 0)) {
             for (int r = 0; r < TY; ++r) {
                 if (!(redo & (1u << r))) continue;
