@@ -356,13 +356,31 @@ __device__ __forceinline__ double local_tau(const KP& P, const double (&p)[3]) {
     return s > 1e-30 ? (w0 * P.tau[0] + w1 * P.tau[1] + w2 * P.tau[2]) / s : P.tau_min;
 }
 
-// project_simplex :491-509; false where the reference throws.
-template <class Div>
+// project_simplex :491-509; false where the reference throws.  FINITE: the
+// caller has checked that z is finite, and the descending sort takes max and
+// min instead of the reference's compare-and-swap.  For finite values the
+// two can differ only in the zeros among the sorted values (a +0 against a
+// -0), and those change neither a partial sum's value before 1 is
+// subtracted nor any comparison, so the threshold and the result are the
+// same.
+template <bool FINITE = false, class Div>
 __device__ __forceinline__ bool project_simplex(const double (&z)[3], double r3, double (&out)[3], Div& div) {
     double s0 = z[0], s1 = z[1], s2 = z[2], t;
-    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
-    if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
-    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+    if constexpr (FINITE) {
+        t = s0;
+        s0 = __builtin_fmax(t, s1);
+        s1 = __builtin_fmin(t, s1);
+        t = s1;
+        s1 = __builtin_fmax(t, s2);
+        s2 = __builtin_fmin(t, s2);
+        t = s0;
+        s0 = __builtin_fmax(t, s1);
+        s1 = __builtin_fmin(t, s1);
+    } else {
+        if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+        if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
+        if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+    }
     double sum = 0.0, threshold = 0.0;
     int active = 0;
     sum += s0;
@@ -920,7 +938,7 @@ __device__ __forceinline__ bool propose_cell64(const KP& P, const double* __rest
     double z[3], zp[3], force[3], mean, scale;
     propose_z(P, y, p, qw, qe, ps, pn, m, dt, force, mean, scale, z, south, north, div);
     const bool finite = __builtin_isfinite(z[0]) && __builtin_isfinite(z[1]) && __builtin_isfinite(z[2]);
-    const bool projected = !finite || project_simplex(z, P.r3, zp, div);
+    const bool projected = !finite || project_simplex<true>(z, P.r3, zp, div);
     if constexpr (std::is_same<Div, DivRun>::value) {
         const bool held = div.ok();
         if (__builtin_expect(!(held && commit), 0)) return held;
