@@ -453,9 +453,9 @@ int run_domains(pearl_ctx* c, const double* stop_time, const long long* max_step
         c->sc[d].rejected = h.rejected;
         peak = std::max(peak, h.cg_peak);
     }
-    // PCG slots per round: the largest solve plus the true-residual check
-    // (a longer solve continues in the next round).
-    if (peak > 0) c->cg_slots = std::min(24, std::max(3, peak + 1));
+    // PCG slots per round: the largest solve (its true-residual check runs
+    // inside finalize; a longer solve continues in the next round).
+    if (peak > 0) c->cg_slots = std::min(24, std::max(3, peak));
     return PEARL_OK;
 }
 
@@ -1019,6 +1019,7 @@ int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, d
         HIP_CHECK(c, hipMemcpyAsync(dom_phi(c, 0, h.cur ^ 1), phi_trial, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         h.phase = PH_SOLVE;
         h.live = 1;
+        h.cg_alone = 0;
         h.cg_state = CG_IDLE;
         h.cg_it = 0;
         h.cg_res = 0;
@@ -1114,7 +1115,7 @@ int run_fixed(pearl_ctx* c, const double* dts, int n, double* save = nullptr) {
         batch = std::max<long long>(1, std::min<long long>(64, h.target_steps - h.taken));
     }
     const int code = h.status == ST_OK ? PEARL_OK : status_of(c, 0);
-    if (h.cg_peak > 0) c->cg_slots = std::min(24, std::max(3, h.cg_peak + 1));
+    if (h.cg_peak > 0) c->cg_slots = std::min(24, std::max(3, h.cg_peak));
     h.status = ST_OK;
     c->sc[0].time = h.time;
     c->sc[0].step = h.step;
@@ -1180,6 +1181,7 @@ int copy_state(pearl_ctx* c, double* dst_phi, double* dst_c, double* dst_mu, con
 int adjoint_solve(pearl_ctx* c, const Adj& A, int& slots) {
     DevScal& h = c->hS[0];
     h.phase = PH_SOLVE;
+    h.cg_alone = 1;
     h.cg_state = CG_IDLE;
     h.cg_it = 0;
     h.cg_res = 0;

@@ -70,9 +70,12 @@ struct KP {
 // Controller phases of one domain.
 enum { PH_IDLE = 0, PH_START = 1, PH_PROPOSE = 2, PH_SOLVE = 3, PH_DECIDE = 4 };
 // PCG states (implicit_diffusion :1047-1080).  CG_ITER: take a step;
-// CG_TRUE: recompute the true residual (:1053); CG_OK / CG_FAIL: done;
-// CG_IDLE: not started.
-enum { CG_IDLE = 0, CG_ITER = 1, CG_TRUE = 2, CG_OK = 3, CG_FAIL = 4 };
+// CG_TRUE: recompute the true residual (:1053) in a pass of its own;
+// CG_CHECK: the recursive residual passed, and the finalize pass computes
+// the true residual beside the carbon update it would keep (the update is
+// discarded when the check fails); CG_OK / CG_FAIL: done; CG_IDLE: not
+// started.
+enum { CG_IDLE = 0, CG_ITER = 1, CG_TRUE = 2, CG_OK = 3, CG_FAIL = 4, CG_CHECK = 5 };
 // Device error codes (the C ABI's PEARL_ERR_* values).
 enum { ST_OK = 0, ST_INVALID_TIME = 1, ST_DT_MIN = 2, ST_RETRY_LIMIT = 3, ST_MIXTURE = 4, ST_PROJECTION = 5,
        ST_FIXED_SOLVE = 6 };
@@ -97,6 +100,7 @@ struct DevScal {
     // attempt
     int live;              // the phase increment passed (:1131)
     int cg_state, cg_it, restart, pidx;
+    int cg_alone;          // a solve with no finalize pass after it (the adjoint's): the true residual is a pass of its own
     int err;               // ERRF_* bits
     unsigned counter;      // last-block counter
     double change;
@@ -658,7 +662,7 @@ __device__ __forceinline__ void cg_head(const KP& P, DevScal* S, int it) {
     S->cg_it = it;
     S->cg_res = res;
     if (!__builtin_isfinite(res)) S->cg_state = CG_FAIL;
-    else if (res <= S->tol) S->cg_state = CG_TRUE;
+    else if (res <= S->tol) S->cg_state = S->cg_alone ? CG_TRUE : CG_CHECK;
     else if (it == P.cg_max) S->cg_state = CG_FAIL;
     else S->cg_state = CG_ITER;
     if (S->cg_state == CG_FAIL) S->phase = PH_DECIDE;
@@ -697,6 +701,7 @@ __global__ void k_begin(Bufs B) {
     }
     if (S->phase == PH_PROPOSE) {
         S->live = 0;
+        S->cg_alone = 0;
         S->cg_state = CG_IDLE;
         S->cg_it = 0;
         S->cg_res = 0;
@@ -1476,13 +1481,21 @@ struct Fin {
 
 // F + A: flux-form carbon update, closure, mu reconstruction (:1082-1093),
 // fused with the acceptance sweep on the trial state: valid_state (:1096),
-// total_carbon (:784) and reduced_energy (:793).
+// total_carbon (:784) and reduced_energy (:793).  In state CG_CHECK it also
+// forms the true residual r = rhs - A x of the converged solve (:1053-1055)
+// and sums r.r: every operand is already here (x and K in the tile, chi and
+// the intercept from the trial phases, rhs = c_old - intercept as prepare
+// forms it), so r is the true-residual pass's value bit for bit and the sum
+// keeps its fold order (rows in order per thread, then the same tree).  The
+// finisher keeps the update when the check passes (:1056) and otherwise
+// hands the solve back to the true-residual pass, which recomputes r and z
+// and restarts (:1060-1065); the discarded update is rewritten later.
 template <int TY>
 __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
-    if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
+    if (!(S->phase == PH_SOLVE && (S->cg_state == CG_OK || S->cg_state == CG_CHECK))) return;
     record(B, slot, w.lb, 1);
     const int n = P.n;
     const int cur = S->cur;
@@ -1493,7 +1506,7 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     const double* __restrict__ c_old = B.c[cur] + P.off;
     double* __restrict__ c_new = B.c[cur ^ 1] + P.off;
     double* __restrict__ mu_new = B.mu[cur ^ 1] + P.off;
-    double v[4] = {0.0, 1.0, 0.0, 0.0};
+    double v[5] = {0.0, 1.0, 0.0, 0.0, 0.0};
     bool good = true, mix_ok = true;
     const Col col = column<TY>(P, w.lb, P.ny);
     // LDS tile: x and K of the tile and its halo; the trial phases march
@@ -1519,12 +1532,16 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
             const double xi = L.a[r][cc];
             const double change = dt * (wf[0] * (L.a[r][cc - 1] - xi) + wf[1] * (L.a[r][cc + 1] - xi) +
                                         wf[2] * (L.a[r - 1][cc] - xi) + wf[3] * (L.a[r + 1][cc] - xi));
-            const double cn = ld_once(c_old + i) + change;
+            const double co = ld_once(c_old + i);
+            const double cn = co + change;
             // the intercept and susceptibility prepare formed from the same
             // trial phases: the same expressions, hence the same values
             Mix m;
             if (!mixture(P, c.p, m)) mix_ok = false;
             const double ic = composition(P.mu_ref, m), ch = m.s;
+            const double rt = (co - ic) - apply_vals(ch, dt, wf, xi, L.a[r][cc - 1], L.a[r][cc + 1], L.a[r - 1][cc],
+                                                     L.a[r + 1][cc]);
+            v[4] += rt * rt;
             v[0] = dmaxd(v[0], __builtin_fabs(cn - (ic + ch * xi)));
             const double mn = P.mu_ref + (cn - ic) / ch;
             st_once(c_new + i, cn);
@@ -1537,17 +1554,27 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
             v[3] += e;
         });
     v[1] = good ? 1.0 : 0.0;
-    block_partial<4, (1u | (2u << 2))>(v, B.part + P.poff, P.nb, w.lb);
+    block_partial<5, (1u | (2u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
 
 __global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
     const int d = blockIdx.x;
     const KP P = B.kp[d];
     DevScal* S = B.S + d;
-    if (!(S->phase == PH_SOLVE && S->cg_state == CG_OK)) return;
-    double v[4];
-    if (!final_reduce<4, (1u | (2u << 2))>(v, B.part + P.poff, P.nb)) return;
+    if (!(S->phase == PH_SOLVE && (S->cg_state == CG_OK || S->cg_state == CG_CHECK))) return;
+    double v[5];
+    if (!final_reduce<5, (1u | (2u << 2))>(v, B.part + P.poff, P.nb)) return;
     S->runs[RUN_FINALIZE]++;
+    if (S->cg_state == CG_CHECK) {
+        S->rr = v[4];
+        const double res = __builtin_sqrt(v[4]);
+        S->cg_res = res;
+        if (!(res <= S->tol * 1.05)) {
+            S->cg_state = CG_TRUE;
+            return;
+        }
+        S->cg_state = CG_OK;
+    }
     S->closure = v[0];
     S->valid = v[1];
     S->mass = v[2];
