@@ -1363,6 +1363,11 @@ __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
         S->cg_res = res;
         if (res <= S->tol * 1.05) {
             S->cg_state = CG_OK;
+        } else if (S->cg_it >= P.cg_max) {
+            // the restart's `continue` (:1065) leaves the loop after its
+            // last iteration: not converged, info.it = cg_max
+            S->cg_state = CG_FAIL;
+            S->phase = PH_DECIDE;
         } else {
             S->rz = v[1];
             S->restart = 1;

@@ -103,6 +103,69 @@ def test_implicit_diffusion(pkg, O, name, k):
         assert np.abs(d["carbon"] - g["diff_carbon"]).max() <= 1e-14
 
 
+# (fixture, cg_rtol, k): with cg_atol = 0 and cg_rtol near the rounding floor
+# the recursive residual passes the tolerance while the true residual does
+# not (:1053-1065): the solve restarts from the true residual once and then
+# converges, or (the last case) restarts until cg_max and fails.  Counted
+# with an instrumented copy of the oracle: one failed check in each of the
+# first five cases, 997 in the last.
+RESTART_CASES = [("small_48x40", 1e-16, 1), ("small_48x40", 1e-16, 4), ("colony_256x128", 1e-16, 4),
+                 ("ragged_257x9", 2e-16, 4), ("window_alpha_32x40", 1e-16, 1), ("colony_256x128", 5e-17, 4)]
+
+
+@pytest.mark.parametrize("name,rtol,k", RESTART_CASES)
+def test_true_residual_check_failure(pkg, O, name, rtol, k):
+    """The true-residual check that finalize carries: where it fails, the
+    carbon update it formed is dropped, the true-residual pass recomputes r
+    and z and the solve restarts.  Result, iteration count and residual
+    bitwise the oracle's in the same blocked order; the separate pass ran."""
+    g = load_golden(name)
+    pd = dict(g["pd"])
+    pd["cg_rtol"], pd["cg_atol"] = rtol, 0.0
+    say("true residual check failure", name, rtol, "k", k)
+    with pkg.Solver(pd, float(g["sc0"][9]), cells_per_thread=k) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        d = s.implicit_diffusion(g["propose_phi"], float(g["dt_probe"]))
+        passes = s.kernel_stats()["true_residual"]["launches"]
+    P = O.params_from_dict(pd, red_mode=O.RED_BLOCKED, red_k=k)
+    o = O.implicit_diffusion(P, g["propose_phi"], g["c0"], g["mu0"], float(g["dt_probe"]), float(g["sc0"][9]))
+    assert (d["ok"], d["it"]) == (o["ok"], o["it"])
+    assert d["residual"] == o["residual"] and d["closure"] == o["closure"]
+    assert passes >= 1
+    if o["ok"]:
+        assert np.array_equal(d["carbon"], o["carbon"]) and np.array_equal(d["mu"], o["mu"])
+    else:
+        assert o["it"] == int(pd["cg_max"])
+
+
+def test_true_residual_check_failure_in_steps(pkg, O):
+    """Whole accepted steps whose solves restart after a failed true-residual
+    check (cg_rtol 1e-16, cg_atol 0): step infos, fields and scalars bitwise
+    the oracle's."""
+    g = load_golden("small_48x40")
+    pd = dict(g["pd"])
+    pd["cg_rtol"], pd["cg_atol"] = 1e-16, 0.0
+    end, steps = float(g["end_time"]), 4
+    say("true residual check failure in steps")
+    with pkg.Solver(pd, float(g["sc0"][9])) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        k = s.layout()[1]
+        infos = [s.advance(end) for _ in range(steps)]
+        phi, c, mu, sc = s.get_state()
+        passes = s.kernel_stats()["true_residual"]["launches"]
+    P = O.params_from_dict(pd, red_mode=O.RED_BLOCKED, red_k=k)
+    S = O.OracleState(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+    for kk in range(steps):
+        rc, oi = O.advance(P, S, float(g["sc0"][9]), end)
+        assert rc == 0
+        gi = infos[kk]
+        assert [gi.accepted_dt, gi.linear_residual, gi.closure, gi.cg_iterations, gi.retries] == \
+            [oi.accepted_dt, oi.linear_residual, oi.closure, oi.cg_iterations, oi.retries], kk
+    assert passes >= 1
+    assert np.array_equal(phi, S.phi) and np.array_equal(c, S.carbon) and np.array_equal(mu, S.mu)
+    assert sc.to_list() == S.scalars()
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_state_reductions(pkg, O, name):
     g = load_golden(name)
