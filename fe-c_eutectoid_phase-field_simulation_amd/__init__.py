@@ -221,9 +221,11 @@ def stream_probe(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> floa
 
 
 # FP64 planes each cell kernel reads and writes once per cell (the access
-# pattern of its algorithmic bytes; the FP64 plain form).
-KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 7), "cg_apply": (4, 2), "cg_update": (5, 3),
-                  "true_residual": (5, 2), "finalize_accept": (6, 2)}
+# pattern of its algorithmic bytes; the FP64 plain form).  cg_apply: its
+# launches that also apply the previous step to x (all but the first of a
+# solve); finalize_accept: with the last step applied.
+KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 5), "cg_apply": (5, 3), "cg_update": (3, 2),
+                  "true_residual": (5, 2), "finalize_accept": (7, 2)}
 
 
 def stream_mix(reads: int, writes: int, nx: int, ny: int, rows_per_thread: int = 4, reps: int = 20,

@@ -128,6 +128,8 @@ __global__ __launch_bounds__(TPB) void k_adj_prepare_fin(Bufs B) {
     S->tol = P.cg_atol + P.cg_rtol * __builtin_sqrt(v[0]);
     S->restart = 1;
     S->pad = 0;
+    S->x_mu = 0;
+    S->x_lag = 0;
     cg_head(P, S, 0);
 }
 

@@ -35,20 +35,31 @@ const char* kKernelNames[PEARL_KERNEL_COUNT] = {
     "propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
     "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"};
 
-// Algorithmic bytes per cell of each kernel (each logical array counted
-// once per pass): propose R phi(3),mu W phi'(3); prepare R phi'(3),c,mu
-// W chi,K,rhs,x,diag,r,z; apply R z,p_old,K,chi W p,Ap; update
-// R x,p,r,Ap,diag W x,r,z; true residual R x,K,chi,rhs,diag W r,z;
-// finalize+accept R phi'(3),x,K,c W c',mu'; state reduce R phi(3),c,mu.
-// Plain form: 8 B per value.  Mixed form: diag, z and p are 4 B.
-const double kBytes[2][PEARL_KERNEL_COUNT] = {
-    {56, 96, 48, 64, 56, 0, 64, 40, 0, 0},
-    {56, 88, 36, 52, 48, 0, 64, 40, 0, 0},
+// Algorithmic bytes per cell of the work each device run counter counts
+// (each logical array once per pass), the kernel id it is reported under,
+// and whether it counts launches (RUN_FIN_LAG adds the bytes of a finalize
+// launch already counted).  Plain form: 8 B per value; mixed form: diag, z
+// and p are 4 B.
+//   propose             R phi(3),mu             W phi'(3)
+//   prepare             R phi'(3),c,mu          W chi,K,diag,r,z
+//   apply (with x step) R z,p_old,K,chi,x       W p,Ap,x     (x: mu before its first store)
+//   apply (fresh)       R z,K,chi               W p,Ap
+//   update              R r,Ap,diag             W r,z        (the adjoint's also R x,p W x)
+//   true residual       R x,K,chi,rhs,diag      W r,z        (the adjoint's; the forward's forms rhs)
+//   finalize+accept     R phi'(3),x,K,c         W c',mu'     (+ R p when it applies the last step)
+//   state reduce        R phi(3),c,mu
+struct RunBytes {
+    int kid;
+    double bytes[2];
+    bool launch;
 };
-
-// Device run counter of each kernel id (-1: none).
-const int kRunOf[PEARL_KERNEL_COUNT] = {RUN_PROPOSE,       RUN_PREPARE, RUN_CG_APPLY, RUN_CG_UPDATE,
-                                        RUN_TRUE_RESIDUAL, -1,          RUN_FINALIZE, RUN_STATE, -1, -1};
+const RunBytes kRunBytes[RUN_COUNT] = {
+    {PEARL_KERNEL_PROPOSE, {56, 56}, true},       {PEARL_KERNEL_PREPARE, {80, 72}, true},
+    {PEARL_KERNEL_CG_APPLY, {64, 52}, true},      {PEARL_KERNEL_CG_UPDATE, {40, 32}, true},
+    {PEARL_KERNEL_TRUE_RESIDUAL, {56, 48}, true}, {PEARL_KERNEL_FINALIZE, {64, 64}, true},
+    {PEARL_KERNEL_STATE_REDUCE, {40, 40}, true},  {PEARL_KERNEL_CG_APPLY, {40, 32}, true},
+    {PEARL_KERNEL_FINALIZE, {8, 4}, false},
+};
 
 }  // namespace
 
@@ -242,7 +253,7 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
         ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { launch_cg_b(c, g, B, s); },
            [&] { hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B); });
     }
-    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_finalize, g, B, s); },
+    ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH_P(c, k_finalize, g, B, s); },
        [&] { hipLaunchKernelGGL(k_finalize_fin, gf, tb, 0, st, B); });
     if (c->any_phase_only)
         ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
@@ -286,13 +297,12 @@ void harvest(pearl_ctx* c) {
     }
     // bytes from the device work counters (exact per domain)
     for (int d = 0; d < c->ndom; ++d) {
-        for (int kid = 0; kid < PEARL_KERNEL_COUNT; ++kid) {
-            const int r = kRunOf[kid];
-            if (r < 0) continue;
+        for (int r = 0; r < RUN_COUNT; ++r) {
+            const RunBytes& rb = kRunBytes[r];
             const unsigned now = c->hS[d].runs[r];
             unsigned& seen = c->runs_seen[static_cast<size_t>(d) * NKIND + r];
-            c->bytes[kid] += kBytes[c->mx ? 1 : 0][kid] * c->kp[d].n * static_cast<double>(now - seen);
-            if (!c->prof) c->launches[kid] += now - seen;
+            c->bytes[rb.kid] += rb.bytes[c->mx ? 1 : 0] * c->kp[d].n * static_cast<double>(now - seen);
+            if (!c->prof && rb.launch) c->launches[rb.kid] += now - seen;
             seen = now;
         }
     }

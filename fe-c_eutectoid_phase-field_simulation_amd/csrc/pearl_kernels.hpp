@@ -81,8 +81,11 @@ enum { ST_OK = 0, ST_INVALID_TIME = 1, ST_DT_MIN = 2, ST_RETRY_LIMIT = 3, ST_MIX
        ST_FIXED_SOLVE = 6 };
 enum { ERRF_MIXTURE = 1, ERRF_PROJECTION = 2, ERRF_NONFINITE_CARBON = 4, ERRF_MIXTURE_STATE = 8 };
 // Kernels counted per domain (work actually done).
+// RUN_CG_APPLY: PCG direction launches that also applied the previous step
+// to the solution; RUN_CG_FRESH: those that did not (the first of a solve);
+// RUN_FIN_LAG: finalize launches that applied the last step themselves.
 enum { RUN_PROPOSE = 0, RUN_PREPARE, RUN_CG_APPLY, RUN_CG_UPDATE, RUN_TRUE_RESIDUAL, RUN_FINALIZE, RUN_STATE,
-       RUN_COUNT };
+       RUN_CG_FRESH, RUN_FIN_LAG, RUN_COUNT };
 
 // Device-resident state of one domain: controller, step scalars, the
 // results of the reductions and the StepInfo of the last accepted step.
@@ -101,6 +104,10 @@ struct DevScal {
     int live;              // the phase increment passed (:1131)
     int cg_state, cg_it, restart, pidx;
     int cg_alone;          // a solve with no finalize pass after it (the adjoint's): the true residual is a pass of its own
+    // The forward solve's x = mu - mu_ref: x_mu, the plane is not written yet
+    // and x is mu_old - mu_ref; x_lag, x still lacks alpha p of the last
+    // step (p in buffer pidx), which the next kernel that needs x applies.
+    int x_mu, x_lag;
     int err;               // ERRF_* bits
     unsigned counter;      // last-block counter
     double change;
@@ -1135,7 +1142,10 @@ struct Prep {
 // comes from the column registers (south/north) or from the trial phases of
 // the west/east cell (the same expression, hence the same bits).  Sums
 // rhs.rhs, r.z, r.r and flags an invalid mixture; the finisher forms the
-// tolerance and evaluates the loop head for iteration 0.
+// tolerance and evaluates the loop head for iteration 0.  Neither rhs nor
+// the initial x = mu_old - mu_ref is stored: the kernels that need them
+// form them again from c_old, the trial phases and mu_old, with the same
+// expressions.
 template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     using T = Coef<MX>;
@@ -1152,8 +1162,6 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     const double* __restrict__ mu_old = B.mu[cur] + P.off;
     double* __restrict__ chi = B.chi + P.off;
     double* __restrict__ K = B.K + P.off;
-    double* __restrict__ rhs = B.rhs + P.off;
-    double* __restrict__ x = B.x + P.off;
     T* __restrict__ diag = reinterpret_cast<T*>(B.diag) + P.off;
     double* __restrict__ r = B.r + P.off;
     T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
@@ -1204,8 +1212,6 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
         const double kc = L.b[row][cc], xc = L.a[row][cc];
         st_once(chi + i, ch);
         st_once(K + i, kc);
-        st_once(rhs + i, b);
-        st_once(x + i, xc);
         double wf[4];
         face_weights(P, y, kc, L.b[row][cc - 1], L.b[row][cc + 1], L.b[row - 1][cc], L.b[row + 1][cc], wf);
         const double ax =
@@ -1242,7 +1248,33 @@ __global__ __launch_bounds__(TPB) void k_prepare_fin(Bufs B) {
     S->tol = P.cg_atol + P.cg_rtol * __builtin_sqrt(v[0]);
     S->restart = 1;
     S->pad = 0;
+    S->x_mu = 1;
+    S->x_lag = 0;
     cg_head(P, S, 0);
+}
+
+// The forward solve's solution at cell j as the kernel after the last
+// update sees it (DevScal::x_mu, x_lag): the x plane, or mu_old - mu_ref
+// before the plane is first written (prepare's expression), plus the last
+// step alpha p where it is still to be applied (cg_update's expression).
+template <bool MU, bool LAG, class T>
+__device__ __forceinline__ double x_at(const double* __restrict__ x, const double* __restrict__ mu_old, double mu_ref,
+                                       const T* __restrict__ p, double alpha, int j) {
+    const double b = MU ? mu_old[j] - mu_ref : x[j];
+    return LAG ? b + alpha * static_cast<double>(p[j]) : b;
+}
+
+// f(MU, LAG) with the two flags as compile-time constants (uniform per
+// domain, so each kernel runs one branch-free form).
+template <class F>
+__device__ __forceinline__ void with_x_form(int mu, int lag, F&& f) {
+    if (mu) {
+        if (lag) f(std::true_type{}, std::true_type{});
+        else f(std::true_type{}, std::false_type{});
+    } else {
+        if (lag) f(std::false_type{}, std::true_type{});
+        else f(std::false_type{}, std::false_type{});
+    }
 }
 
 // Search direction and diffusivity of one cell.
@@ -1258,9 +1290,14 @@ struct SolK {
 // PCG kernel A.  In state CG_ITER: the search direction of this iteration
 // p = z + beta p_old (:1079; p = z after a (re)start :1042/:1062), kept in
 // the other p buffer, then Ap = A p (:1068) and p.Ap; the finisher checks
-// for breakdown (:1070) and forms the step length (:1071).  In state
-// CG_TRUE: the true residual r = rhs - A x with z = r/diag (:1053-1062);
-// the finisher accepts (:1056) or restarts from it (:1060-1065).
+// for breakdown (:1070) and forms the step length (:1071).  In the forward
+// solve it also applies the previous step's x += alpha p_old (:1074), which
+// cg_update leaves to it: this kernel reads p_old anyway, so x costs its
+// read and write here instead of x and p reads and the x write there.  In
+// state CG_TRUE: the true residual r = rhs - A x with z = r/diag
+// (:1053-1062), rhs formed from c_old and the trial phases as prepare forms
+// it (the adjoint's solve stores its rhs); the finisher accepts (:1056) or
+// restarts from it (:1060-1065).
 template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     using T = Coef<MX>;
@@ -1273,12 +1310,15 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const double* __restrict__ chi = B.chi + P.off;
     const double* __restrict__ K = B.K + P.off;
     T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
+    double* __restrict__ x = B.x + P.off;
+    const double* __restrict__ mu_old = B.mu[S->cur] + P.off;
+    const int pi = S->pidx;
+    const T* __restrict__ po = reinterpret_cast<const T*>(pi ? B.p1 : B.p0) + P.off;
+    const double alpha = S->alpha;  // the previous step's (its finisher has not run yet)
     const Col col = column<TY>(P, w.lb, P.ny);
     __shared__ TileLds<TY> L;  // a: the stencil variable, b: K
     if (state == CG_ITER) {
         record(B, slot, w.lb, 1);
-        const int pi = S->pidx;
-        const T* __restrict__ po = reinterpret_cast<const T*>(pi ? B.p1 : B.p0) + P.off;
         T* __restrict__ pn = reinterpret_cast<T*>(pi ? B.p0 : B.p1) + P.off;
         double* __restrict__ Ap = B.Ap + P.off;
         const double beta = S->beta;
@@ -1296,34 +1336,53 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         if (S->restart != 0) stage(std::true_type{});
         else stage(std::false_type{});
         __syncthreads();
-        for_tile<TY>(P, col, [&](const int i, const int y, const int r, const int cc) {
-            double wf[4];
-            face_weights(P, y, L.b[r][cc], L.b[r][cc - 1], L.b[r][cc + 1], L.b[r - 1][cc], L.b[r + 1][cc], wf);
-            const double c = L.a[r][cc];
-            const double a = apply_vals(ld_once(chi + i), dt, wf, c, L.a[r][cc - 1], L.a[r][cc + 1], L.a[r - 1][cc],
-                                        L.a[r + 1][cc]);
-            st_once(pn + i, static_cast<T>(c));
-            st_once(Ap + i, a);
-            v[0] += c * a;
-        });
+        auto body = [&](auto mu, auto lag) {
+            for_tile<TY>(P, col, [&](const int i, const int y, const int r, const int cc) {
+                double wf[4];
+                face_weights(P, y, L.b[r][cc], L.b[r][cc - 1], L.b[r][cc + 1], L.b[r - 1][cc], L.b[r + 1][cc], wf);
+                const double c = L.a[r][cc];
+                const double a = apply_vals(ld_once(chi + i), dt, wf, c, L.a[r][cc - 1], L.a[r][cc + 1],
+                                            L.a[r - 1][cc], L.a[r + 1][cc]);
+                st_once(pn + i, static_cast<T>(c));
+                st_once(Ap + i, a);
+                if constexpr (decltype(lag)::value)
+                    st_once(x + i, x_at<decltype(mu)::value, true>(x, mu_old, P.mu_ref, po, alpha, i));
+                v[0] += c * a;
+            });
+        };
+        with_x_form(S->x_mu, S->x_lag, body);
         block_partial<1, 0u>(v, B.part + P.poff, P.nb, w.lb);
     } else if (state == CG_TRUE) {
         record(B, slot, w.lb, 2);
         const double* __restrict__ rhs = B.rhs + P.off;
-        const double* __restrict__ x = B.x + P.off;
+        const double* __restrict__ c_old = B.c[S->cur] + P.off;
+        const double* __restrict__ phi_t = B.phi[S->cur ^ 1] + 3 * P.off;
         const T* __restrict__ diag = reinterpret_cast<const T*>(B.diag) + P.off;
         double* __restrict__ r = B.r + P.off;
+        const bool alone = S->cg_alone != 0;
         double v[2] = {0.0, 0.0};
-        stage_tile<TY>(P, col, [&](const int rr, const int cc, const int j) {
-            L.a[rr][cc] = x[j];
-            L.b[rr][cc] = K[j];
+        with_x_form(S->x_mu, S->x_lag, [&](auto mu, auto lag) {
+            stage_tile<TY>(P, col, [&](const int rr, const int cc, const int j) {
+                L.a[rr][cc] = x_at<decltype(mu)::value, decltype(lag)::value>(x, mu_old, P.mu_ref, po, alpha, j);
+                L.b[rr][cc] = K[j];
+            });
         });
         __syncthreads();
         for_tile<TY>(P, col, [&](const int i, const int y, const int rw, const int cc) {
             double wf[4];
             face_weights(P, y, L.b[rw][cc], L.b[rw][cc - 1], L.b[rw][cc + 1], L.b[rw - 1][cc], L.b[rw + 1][cc], wf);
-            const double rr = ld_once(rhs + i) - apply_vals(ld_once(chi + i), dt, wf, L.a[rw][cc], L.a[rw][cc - 1],
-                                                            L.a[rw][cc + 1], L.a[rw - 1][cc], L.a[rw + 1][cc]);
+            double b;
+            if (alone) {
+                b = ld_once(rhs + i);
+            } else {
+                double q[3];
+                load_phi(phi_t, P.n, i, q);
+                Mix m;
+                (void)mixture(P, q, m);
+                b = ld_once(c_old + i) - composition(P.mu_ref, m);
+            }
+            const double rr = b - apply_vals(ld_once(chi + i), dt, wf, L.a[rw][cc], L.a[rw][cc - 1], L.a[rw][cc + 1],
+                                             L.a[rw - 1][cc], L.a[rw + 1][cc]);
             const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
             st_once(r + i, rr);
             st_once(z + i, static_cast<T>(zz));
@@ -1343,7 +1402,13 @@ __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
     if (state == CG_ITER) {
         double v[1];
         if (!final_reduce<1, 0u>(v, B.part + P.poff, P.nb)) return;
-        S->runs[RUN_CG_APPLY]++;
+        if (S->x_lag) {
+            S->runs[RUN_CG_APPLY]++;
+            S->x_lag = 0;
+            S->x_mu = 0;
+        } else {
+            S->runs[RUN_CG_FRESH]++;
+        }
         const double pAp = v[0], rz = S->rz;
         S->pAp = pAp;
         if (!(pAp > 0) || !__builtin_isfinite(pAp) || !__builtin_isfinite(rz)) {
@@ -1376,9 +1441,11 @@ __global__ __launch_bounds__(TPB) void k_cg_a_fin(Bufs B) {
     }
 }
 
-// PCG kernel B, after a step of kernel A: x += alpha p; r -= alpha Ap;
-// z = r/diag; sums r.z and r.r; the finisher forms beta (:1077) and
-// evaluates the loop head of the next iteration.
+// PCG kernel B, after a step of kernel A: r -= alpha Ap; z = r/diag
+// (:1074-1076); sums r.z and r.r; the finisher forms beta (:1077) and
+// evaluates the loop head of the next iteration.  x += alpha p (:1074) is
+// left to the next kernel that reads x or p (DevScal::x_lag), except in the
+// adjoint's solve, whose result is the x plane itself.
 // Threads per column of cg_update.  Its cells are independent, so two
 // threads share a column's rows and twice as many loads are in flight; the
 // fold stays the column's sequential one: the thread of the upper rows hands
@@ -1403,6 +1470,7 @@ __global__ __launch_bounds__(TPB * cgb_split<TY>()) void k_cg_b(Bufs B, int slot
     const double* __restrict__ Ap = B.Ap + P.off;
     const T* __restrict__ diag = reinterpret_cast<const T*>(B.diag) + P.off;
     const double alpha = S->alpha;
+    const bool upd_x = S->cg_alone != 0;
     const int t = static_cast<int>(threadIdx.x) % TPB, h = static_cast<int>(threadIdx.x) / TPB;
     const Col col = column<TY>(P, w.lb, P.ny, t);
     double prod[RH][2];
@@ -1416,7 +1484,7 @@ __global__ __launch_bounds__(TPB * cgb_split<TY>()) void k_cg_b(Bufs B, int slot
             // nontemporal throughout: measured against cg_apply run alternately
             // with it, the pair is 9 % faster than with plain accesses (plain
             // stores leave dirty lines that the next cg_apply pays for)
-            st_once(x + i, ld_once(x + i) + alpha * static_cast<double>(ld_once(p + i)));
+            if (upd_x) st_once(x + i, ld_once(x + i) + alpha * static_cast<double>(ld_once(p + i)));
             const double rr = ld_once(r + i) - alpha * ld_once(Ap + i);
             const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
             st_once(r + i, rr);
@@ -1476,6 +1544,7 @@ __global__ __launch_bounds__(TPB) void k_cg_b_fin(Bufs B) {
     S->rr = v[1];
     S->restart = 0;
     S->pad = 0;
+    S->x_lag = S->cg_alone ? 0 : 1;
     cg_head(P, S, S->cg_it + 1);
 }
 
@@ -1494,9 +1563,13 @@ struct Fin {
 // keeps its fold order (rows in order per thread, then the same tree).  The
 // finisher keeps the update when the check passes (:1056) and otherwise
 // hands the solve back to the true-residual pass, which recomputes r and z
-// and restarts (:1060-1065); the discarded update is rewritten later.
-template <int TY>
+// and restarts (:1060-1065); the discarded update is rewritten later.  x is
+// staged as the last update left it: with the last step alpha p applied
+// here (cg_update's expression) and, after a solve of at most one step,
+// from mu_old (prepare's expression).
+template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
+    using T = Coef<MX>;
     const Where w = where(B, -1);
     const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
@@ -1508,6 +1581,9 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     const double* __restrict__ phi_t = B.phi[cur ^ 1] + 3 * P.off;
     const double* __restrict__ K = B.K + P.off;
     const double* __restrict__ x = B.x + P.off;
+    const double* __restrict__ mu_old = B.mu[cur] + P.off;
+    const T* __restrict__ p_last = reinterpret_cast<const T*>(S->pidx ? B.p1 : B.p0) + P.off;
+    const double alpha = S->alpha;
     const double* __restrict__ c_old = B.c[cur] + P.off;
     double* __restrict__ c_new = B.c[cur ^ 1] + P.off;
     double* __restrict__ mu_new = B.mu[cur ^ 1] + P.off;
@@ -1517,9 +1593,11 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     // LDS tile: x and K of the tile and its halo; the trial phases march
     // down the column in registers (the energy needs east and north)
     __shared__ TileLds<TY> L;
-    stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
-        L.a[r][cc] = x[j];
-        L.b[r][cc] = K[j];
+    with_x_form(S->x_mu, S->x_lag, [&](auto mu, auto lag) {
+        stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
+            L.a[r][cc] = x_at<decltype(mu)::value, decltype(lag)::value>(x, mu_old, P.mu_ref, p_last, alpha, j);
+            L.b[r][cc] = K[j];
+        });
     });
     __syncthreads();
     const int cc = static_cast<int>(threadIdx.x) + 1;
@@ -1570,6 +1648,7 @@ __global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
     double v[5];
     if (!final_reduce<5, (1u | (2u << 2))>(v, B.part + P.poff, P.nb)) return;
     S->runs[RUN_FINALIZE]++;
+    if (S->x_lag) S->runs[RUN_FIN_LAG]++;
     if (S->cg_state == CG_CHECK) {
         S->rr = v[4];
         const double res = __builtin_sqrt(v[4]);
