@@ -20,10 +20,12 @@ Prints one JSON line:
             a second region of --kernel-steps steps (500) right after the
             timed one (the events add graph nodes and slow the whole loop by
             ~15 %, so the timed region runs without them); `traffic` is the
-            kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE and
-            WRITE_SIZE scaled by a 1 GiB copy kernel) from the committed
-            counter summary of the same workload and tile height, when there
-            is one, and traffic_GBps that traffic over the live launch time;
+            kernel's HBM bytes per launch: its algorithmic bytes per launch
+            here times the measured traffic per algorithmic byte (rocprofv3
+            FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy kernel) from the
+            committed counter summary of the same workload and tile height,
+            when there is one, and traffic_GBps that traffic over the live
+            launch time;
             copy_ceiling_GBps is a hand-written 1 GiB device copy measured in
             the same run (pearl_stream_probe), frac_of_copy the dominant
             kernel's rate against it; for every cell kernel,
@@ -854,10 +856,15 @@ def main():
             with open(measured) as fh:
                 tr = json.load(fh)["kernels"].get(dom[0])
             if tr:
-                roof["traffic"] = round(tr["traffic_bytes"])
+                # the counter run's traffic per algorithmic byte, applied to
+                # this run's launches (a kernel with several launch types has
+                # another mix of them here)
+                ratio = tr["traffic_bytes"] / tr["algorithmic_bytes"]
+                roof["traffic"] = round(ratio * roof["bytes_per_launch"])
+                roof["traffic_over_algorithmic"] = round(ratio, 4)
                 roof["traffic_source"] = os.path.relpath(measured, ROOT)
                 # measured HBM rate (counter bytes per launch / launch time)
-                roof["traffic_GBps"] = round(tr["traffic_bytes"] / (roof["avg_launch_ms"] * 1e-3) / 1e9, 1)
+                roof["traffic_GBps"] = round(roof["traffic"] / (roof["avg_launch_ms"] * 1e-3) / 1e9, 1)
                 roof["traffic_frac"] = round(roof["traffic_GBps"] / HBM_PEAK_GBS, 4)
     total_bytes = sum(v["bytes"] for v in counted.values())
     out = {

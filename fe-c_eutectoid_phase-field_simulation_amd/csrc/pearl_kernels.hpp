@@ -972,10 +972,12 @@ __device__ __forceinline__ bool propose_cell64(const KP& P, const double* __rest
 // LDS tile of a cell kernel: TY rows of the block's 256 columns, the row
 // above and the row below (clamped at the domain's bottom and top rows, where
 // a missing neighbour is the cell itself), and one column either side
-// (periodic in x).  stage_tile calls put(r, cc, j) for LDS row r, LDS column
-// cc and cell j; each thread stages its own column, threads 0 and 255 the
-// two halo columns.  The tile is formed once per block, so a neighbour's
-// value is neither loaded again nor recomputed by the cells around it.
+// (periodic in x).  stage_tile calls put(r, cc, j, own) for LDS row r, LDS
+// column cc and cell j; each thread stages its own column (own: a
+// compile-time true; its rows 1..TY are the thread's cells), threads 0 and
+// 255 the two halo columns.  The tile is formed once per block, so a
+// neighbour's value is neither loaded again nor recomputed by the cells
+// around it.
 template <int TY>
 struct TileLds {
     double a[TY + 2][TPB + 2], b[TY + 2][TPB + 2];
@@ -985,18 +987,18 @@ template <int TY, class F>
 __device__ __forceinline__ void stage_tile(const KP& P, const Col& col, F&& put) {
     const int t = static_cast<int>(threadIdx.x);
     const int x0 = col.x - t;
-    auto one = [&](const int cc, int gx) {
+    auto one = [&](const int cc, int gx, auto own) {
         gx = gx < 0 ? gx + P.nx : (gx >= P.nx ? gx % P.nx : gx);
 #pragma unroll
         for (int r = 0; r < TY + 2; ++r) {
             int y = col.y0 - 1 + r;
             y = y < 0 ? 0 : (y > P.ny - 1 ? P.ny - 1 : y);
-            put(r, cc, y * P.nx + gx);
+            put(r, cc, y * P.nx + gx, own);
         }
     };
-    one(t + 1, x0 + t);
-    if (t == 0) one(0, x0 - 1);
-    if (t == TPB - 1) one(TPB + 1, x0 + TPB);
+    one(t + 1, x0 + t, std::true_type{});
+    if (t == 0) one(0, x0 - 1, std::false_type{});
+    if (t == TPB - 1) one(TPB + 1, x0 + TPB, std::false_type{});
 }
 
 // The block's cells in the march order, after staging: body(i, y, r, cc)
@@ -1325,15 +1327,25 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         double v[1] = {0.0};
         // the direction as it is stored; A p is formed from exactly these
         // values.  The restart flag is uniform: one staging loop per value
-        // keeps the p_old loads free of branches.
+        // keeps the p_old loads free of branches.  The thread keeps p_old of
+        // its own cells for the step x += alpha p_old.
+        double pc[TY];
         auto stage = [&](auto fresh) {
-            stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
+            stage_tile<TY>(P, col, [&](const int r, const int cc, const int j, auto own) {
                 const double zj = z[j];
-                L.a[r][cc] = stored<MX>(decltype(fresh)::value ? zj : zj + beta * static_cast<double>(po[j]));
+                if constexpr (decltype(fresh)::value) {
+                    L.a[r][cc] = stored<MX>(zj);
+                } else {
+                    const double pv = static_cast<double>(po[j]);
+                    if constexpr (decltype(own)::value)
+                        if (r >= 1 && r <= TY) pc[r - 1] = pv;
+                    L.a[r][cc] = stored<MX>(zj + beta * pv);
+                }
                 L.b[r][cc] = K[j];
             });
         };
-        if (S->restart != 0) stage(std::true_type{});
+        const bool fresh = S->restart != 0;
+        if (fresh) stage(std::true_type{});
         else stage(std::false_type{});
         __syncthreads();
         auto body = [&](auto mu, auto lag) {
@@ -1345,8 +1357,12 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
                                             L.a[r - 1][cc], L.a[r + 1][cc]);
                 st_once(pn + i, static_cast<T>(c));
                 st_once(Ap + i, a);
-                if constexpr (decltype(lag)::value)
-                    st_once(x + i, x_at<decltype(mu)::value, true>(x, mu_old, P.mu_ref, po, alpha, i));
+                if constexpr (decltype(lag)::value) {
+                    // x + alpha p_old (cg_update's expression), p_old as stored
+                    const double pv = fresh ? static_cast<double>(po[i]) : pc[r - 1];
+                    const double b = decltype(mu)::value ? mu_old[i] - P.mu_ref : ld_once(x + i);
+                    st_once(x + i, b + alpha * pv);
+                }
                 v[0] += c * a;
             });
         };
@@ -1362,7 +1378,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         const bool alone = S->cg_alone != 0;
         double v[2] = {0.0, 0.0};
         with_x_form(S->x_mu, S->x_lag, [&](auto mu, auto lag) {
-            stage_tile<TY>(P, col, [&](const int rr, const int cc, const int j) {
+            stage_tile<TY>(P, col, [&](const int rr, const int cc, const int j, auto) {
                 L.a[rr][cc] = x_at<decltype(mu)::value, decltype(lag)::value>(x, mu_old, P.mu_ref, po, alpha, j);
                 L.b[rr][cc] = K[j];
             });
@@ -1594,7 +1610,7 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     // down the column in registers (the energy needs east and north)
     __shared__ TileLds<TY> L;
     with_x_form(S->x_mu, S->x_lag, [&](auto mu, auto lag) {
-        stage_tile<TY>(P, col, [&](const int r, const int cc, const int j) {
+        stage_tile<TY>(P, col, [&](const int r, const int cc, const int j, auto) {
             L.a[r][cc] = x_at<decltype(mu)::value, decltype(lag)::value>(x, mu_old, P.mu_ref, p_last, alpha, j);
             L.b[r][cc] = K[j];
         });
