@@ -35,7 +35,7 @@ MIXED = 1  # Jacobi diagonal, p, z in FP32; state, operator, residuals, dots and
 FP32_STENCIL = 2  # MIXED plus FP32 arithmetic in the phase proposal's stencil (force, mean, step scale)
 
 KERNELS = ["propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
-           "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"]
+           "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish", "cg_apply_first"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -221,11 +221,12 @@ def stream_probe(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> floa
 
 
 # FP64 planes each cell kernel reads and writes once per cell (the access
-# pattern of its algorithmic bytes; the FP64 plain form).  cg_apply: its
-# launches that also apply the previous step to x (all but the first of a
-# solve); finalize_accept: with the last step applied.
-KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 5), "cg_apply": (5, 3), "cg_update": (3, 2),
-                  "true_residual": (5, 2), "finalize_accept": (7, 2)}
+# pattern of its algorithmic bytes; the FP64 plain form).  cg_apply: the
+# direction launches that also apply the previous step to x (all but the
+# first of a solve, cg_apply_first); finalize_accept: with the last step
+# applied; true_residual: the adjoint's pass.
+KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 5), "cg_apply": (5, 3), "cg_apply_first": (3, 2),
+                  "cg_update": (3, 2), "true_residual": (5, 2), "finalize_accept": (7, 2)}
 
 
 def stream_mix(reads: int, writes: int, nx: int, ny: int, rows_per_thread: int = 4, reps: int = 20,

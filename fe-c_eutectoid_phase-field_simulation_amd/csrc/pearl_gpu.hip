@@ -33,7 +33,7 @@ const char* kWhy[5] = {"phase increment", "linear/closure", "composition/phase b
 
 const char* kKernelNames[PEARL_KERNEL_COUNT] = {
     "propose_phi", "diffusion_prepare", "cg_apply", "cg_update", "true_residual",
-    "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish"};
+    "idle", "finalize_accept", "state_reduce", "moving_window", "reduce_finish", "cg_apply_first"};
 
 // Algorithmic bytes per cell of the work each device run counter counts
 // (each logical array once per pass), the kernel id it is reported under,
@@ -57,7 +57,7 @@ const RunBytes kRunBytes[RUN_COUNT] = {
     {PEARL_KERNEL_PROPOSE, {56, 56}, true},       {PEARL_KERNEL_PREPARE, {80, 72}, true},
     {PEARL_KERNEL_CG_APPLY, {64, 52}, true},      {PEARL_KERNEL_CG_UPDATE, {40, 32}, true},
     {PEARL_KERNEL_TRUE_RESIDUAL, {56, 48}, true}, {PEARL_KERNEL_FINALIZE, {64, 64}, true},
-    {PEARL_KERNEL_STATE_REDUCE, {40, 40}, true},  {PEARL_KERNEL_CG_APPLY, {40, 32}, true},
+    {PEARL_KERNEL_STATE_REDUCE, {40, 40}, true},  {PEARL_KERNEL_CG_FIRST, {40, 32}, true},
     {PEARL_KERNEL_FINALIZE, {8, 4}, false},
 };
 
@@ -278,6 +278,7 @@ void harvest(pearl_ctx* c) {
                 const int kind = e.slot < MAX_SLOTS ? kinds[e.slot] : 0;
                 if (kind == 0) kid = PEARL_KERNEL_CG_IDLE;
                 else if (kid == PEARL_KERNEL_CG_APPLY && kind == 2) kid = PEARL_KERNEL_TRUE_RESIDUAL;
+                else if (kid == PEARL_KERNEL_CG_APPLY && kind == 3) kid = PEARL_KERNEL_CG_FIRST;
             }
             float t = 0;
             (void)hipEventElapsedTime(&t, e.a, e.b);

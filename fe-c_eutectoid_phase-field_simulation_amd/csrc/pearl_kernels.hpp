@@ -1320,7 +1320,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const Col col = column<TY>(P, w.lb, P.ny);
     __shared__ TileLds<TY> L;  // a: the stencil variable, b: K
     if (state == CG_ITER) {
-        record(B, slot, w.lb, 1);
+        record(B, slot, w.lb, S->x_lag ? 1 : 3);
         T* __restrict__ pn = reinterpret_cast<T*>(pi ? B.p0 : B.p1) + P.off;
         double* __restrict__ Ap = B.Ap + P.off;
         const double beta = S->beta;

@@ -339,15 +339,16 @@ int pearl_get_layout(const pearl_ctx* ctx, int* threads_per_block, int* cells_pe
  * PEARL_KERNEL_CG_IDLE with zero bytes. */
 #define PEARL_KERNEL_PROPOSE 0        /* propose_phi */
 #define PEARL_KERNEL_PREPARE 1        /* diffusion setup + initial residual, diagonal, tolerance */
-#define PEARL_KERNEL_CG_APPLY 2       /* direction update + A p + p.Ap */
-#define PEARL_KERNEL_CG_UPDATE 3      /* x, r, z updates + r.z, r.r */
-#define PEARL_KERNEL_TRUE_RESIDUAL 4  /* true residual at convergence (+ restart) */
+#define PEARL_KERNEL_CG_APPLY 2       /* direction update + A p + p.Ap + the previous step x += alpha p */
+#define PEARL_KERNEL_CG_UPDATE 3      /* r, z updates + r.z, r.r */
+#define PEARL_KERNEL_TRUE_RESIDUAL 4  /* true residual after a failed check inside finalize (+ restart) */
 #define PEARL_KERNEL_CG_IDLE 5        /* speculative PCG launches that found the solve finished */
 #define PEARL_KERNEL_FINALIZE 6       /* flux update, closure, mu + acceptance reductions */
 #define PEARL_KERNEL_STATE_REDUCE 7   /* valid/mass/energy of the current state, diagnostics */
 #define PEARL_KERNEL_WINDOW 8         /* moving-window front, strip, shift */
 #define PEARL_KERNEL_FINISH 9         /* one-block finishers of the blocked reductions */
-#define PEARL_KERNEL_COUNT 10
+#define PEARL_KERNEL_CG_FIRST 10      /* first direction of a solve, p = z: A p + p.Ap (no x step) */
+#define PEARL_KERNEL_COUNT 11
 int pearl_profile(pearl_ctx* ctx, int enable);
 int pearl_kernel_stats(pearl_ctx* ctx, int kernel, long long* launches, double* total_ms, double* bytes);
 const char* pearl_kernel_name(int kernel);
