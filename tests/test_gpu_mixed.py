@@ -126,3 +126,31 @@ def test_fp32_stencil_form(pkg):
         assert float(np.max(np.abs(pb.sum(axis=0) - 1.0))) <= 1e-14
         assert abs(b.total_carbon() - m0) <= 1e-12 * abs(m0)
         assert b.valid_state()
+
+
+@pytest.mark.parametrize("name,rtol", [("small_48x40", 1e-16), ("colony_256x128", 1e-16)])
+def test_mixed_solve_after_failed_true_residual_check(pkg, O, name, rtol):
+    """The mixed form's solve where the true-residual check inside finalize
+    fails (cg_atol = 0, cg_rtol near the rounding floor): the separate
+    true-residual pass runs, the solve restarts with the FP32 direction
+    buffers and still converges to the FP64 tolerance.  Bar: the same FP64
+    operator solved to a relative residual of 1e-16, so carbon within 1e-15
+    and mu within 1e-8 J/mol of the oracle's FP64 solve (the residual bound
+    times the largest 1/chi of these fixtures, 5e7 mol/J), and the pass ran."""
+    g = load_golden(name)
+    pd = dict(g["pd"])
+    pd["cg_rtol"], pd["cg_atol"] = rtol, 0.0
+    say("mixed solve after failed check", name, rtol)
+    with pkg.Solver(pd, float(g["sc0"][9]), precision=pkg.MIXED) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        k = s.layout()[1]
+        d = s.implicit_diffusion(g["propose_phi"], float(g["dt_probe"]))
+        passes = s.kernel_stats()["true_residual"]["launches"]
+    P = O.params_from_dict(pd, red_mode=O.RED_BLOCKED, red_k=k)
+    o = O.implicit_diffusion(P, g["propose_phi"], g["c0"], g["mu0"], float(g["dt_probe"]), float(g["sc0"][9]))
+    assert d["ok"] == 1 and o["ok"] == 1
+    assert passes >= 1
+    say("  it", d["it"], "vs", o["it"], "max|dc|", np.abs(d["carbon"] - o["carbon"]).max(),
+        "max|dmu|", np.abs(d["mu"] - o["mu"]).max())
+    assert np.abs(d["carbon"] - o["carbon"]).max() <= 1e-15
+    assert np.abs(d["mu"] - o["mu"]).max() <= 1e-8
