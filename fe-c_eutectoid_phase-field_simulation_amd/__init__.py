@@ -225,7 +225,7 @@ def stream_probe(device: int = 0, nbytes: int = 1 << 30, reps: int = 20) -> floa
 # direction launches that also apply the previous step to x (all but the
 # first of a solve, cg_apply_first); finalize_accept: with the last step
 # applied; true_residual: the adjoint's pass.
-KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 5), "cg_apply": (5, 3), "cg_apply_first": (3, 2),
+KERNEL_STREAMS = {"propose_phi": (4, 3), "diffusion_prepare": (5, 5), "cg_apply": (5, 3), "cg_apply_first": (3, 1),
                   "cg_update": (3, 2), "true_residual": (5, 2), "finalize_accept": (7, 2)}
 
 

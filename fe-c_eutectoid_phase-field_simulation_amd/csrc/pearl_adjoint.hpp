@@ -62,7 +62,8 @@ __device__ __forceinline__ double dh(const double (&p)[3], int hp, int a) {
 // ---------------------------------------------------------------------------
 // Diffusion adjoint, part 1: the adjoint right-hand side gx = gmu' - dt L gc'
 // and the PCG set-up on the trial phases (chi, K, Jacobi diagonal, x = 0,
-// r = gx, z = r/diag); sums rhs.rhs, r.z, r.r for the loop head.
+// r = gx, the first direction p = z = r/diag, stored where the forward's
+// prepare stores it); sums rhs.rhs, r.z, r.r for the loop head.
 // ---------------------------------------------------------------------------
 struct KG {
     double k, g;
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(TPB) void k_adj_prepare(Bufs B, Adj A) {
     double* __restrict__ x = B.x;
     double* __restrict__ diag = B.diag;
     double* __restrict__ r = B.r;
-    double* __restrict__ z = B.z;
+    double* __restrict__ z = B.S->pidx ? B.p0 : B.p1;  // the first direction p = z
     double v[3] = {0.0, 0.0, 0.0};
     const Col col = column<TY>(P, lb, P.ny);
     march<TY, KG>(

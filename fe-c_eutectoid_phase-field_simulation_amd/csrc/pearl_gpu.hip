@@ -41,9 +41,9 @@ const char* kKernelNames[PEARL_KERNEL_COUNT] = {
 // launch already counted).  Plain form: 8 B per value; mixed form: diag, z
 // and p are 4 B.
 //   propose             R phi(3),mu             W phi'(3)
-//   prepare             R phi'(3),c,mu          W chi,K,diag,r,z
+//   prepare             R phi'(3),c,mu          W chi,K,diag,r,p (= z)
 //   apply (with x step) R z,p_old,K,chi,x       W p,Ap,x     (x: mu before its first store)
-//   apply (fresh)       R z,K,chi               W p,Ap
+//   apply (fresh)       R p,K,chi               W Ap
 //   update              R r,Ap,diag             W r,z        (the adjoint's also R x,p W x)
 //   true residual       R x,K,chi,rhs,diag      W r,z        (the adjoint's; the forward's forms rhs)
 //   finalize+accept     R phi'(3),x,K,c         W c',mu'     (+ R p when it applies the last step)
@@ -57,7 +57,7 @@ const RunBytes kRunBytes[RUN_COUNT] = {
     {PEARL_KERNEL_PROPOSE, {56, 56}, true},       {PEARL_KERNEL_PREPARE, {80, 72}, true},
     {PEARL_KERNEL_CG_APPLY, {64, 52}, true},      {PEARL_KERNEL_CG_UPDATE, {40, 32}, true},
     {PEARL_KERNEL_TRUE_RESIDUAL, {56, 48}, true}, {PEARL_KERNEL_FINALIZE, {64, 64}, true},
-    {PEARL_KERNEL_STATE_REDUCE, {40, 40}, true},  {PEARL_KERNEL_CG_FIRST, {40, 32}, true},
+    {PEARL_KERNEL_STATE_REDUCE, {40, 40}, true},  {PEARL_KERNEL_CG_FIRST, {32, 28}, true},
     {PEARL_KERNEL_FINALIZE, {8, 4}, false},
 };
 

@@ -1147,7 +1147,8 @@ struct Prep {
 // tolerance and evaluates the loop head for iteration 0.  Neither rhs nor
 // the initial x = mu_old - mu_ref is stored: the kernels that need them
 // form them again from c_old, the trial phases and mu_old, with the same
-// expressions.
+// expressions.  z = r/diag is the first search direction p (:1042) and is
+// stored as such, in the p buffer that cg_apply's first launch reads.
 template <int TY, bool MX>
 __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     using T = Coef<MX>;
@@ -1166,7 +1167,7 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     double* __restrict__ K = B.K + P.off;
     T* __restrict__ diag = reinterpret_cast<T*>(B.diag) + P.off;
     double* __restrict__ r = B.r + P.off;
-    T* __restrict__ z = reinterpret_cast<T*>(B.z) + P.off;
+    T* __restrict__ z = reinterpret_cast<T*>(S->pidx ? B.p0 : B.p1) + P.off;  // p = z
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const Col col = column<TY>(P, w.lb, P.ny);
     // LDS tile: x = mu_old - mu_ref and K = mixture's K of the trial phases,
@@ -1290,8 +1291,9 @@ struct SolK {
 };
 
 // PCG kernel A.  In state CG_ITER: the search direction of this iteration
-// p = z + beta p_old (:1079; p = z after a (re)start :1042/:1062), kept in
-// the other p buffer, then Ap = A p (:1068) and p.Ap; the finisher checks
+// p = z + beta p_old (:1079), kept in the other p buffer (after a
+// (re)start, p = z :1042/:1062 is already there: prepare or the
+// true-residual pass stored it), then Ap = A p (:1068) and p.Ap; the finisher checks
 // for breakdown (:1070) and forms the step length (:1071).  In the forward
 // solve it also applies the previous step's x += alpha p_old (:1074), which
 // cg_update leaves to it: this kernel reads p_old anyway, so x costs its
@@ -1317,11 +1319,11 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
     const int pi = S->pidx;
     const T* __restrict__ po = reinterpret_cast<const T*>(pi ? B.p1 : B.p0) + P.off;
     const double alpha = S->alpha;  // the previous step's (its finisher has not run yet)
+    T* __restrict__ pn = reinterpret_cast<T*>(pi ? B.p0 : B.p1) + P.off;
     const Col col = column<TY>(P, w.lb, P.ny);
     __shared__ TileLds<TY> L;  // a: the stencil variable, b: K
     if (state == CG_ITER) {
         record(B, slot, w.lb, S->x_lag ? 1 : 3);
-        T* __restrict__ pn = reinterpret_cast<T*>(pi ? B.p0 : B.p1) + P.off;
         double* __restrict__ Ap = B.Ap + P.off;
         const double beta = S->beta;
         double v[1] = {0.0};
@@ -1332,41 +1334,42 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
         double pc[TY];
         auto stage = [&](auto fresh) {
             stage_tile<TY>(P, col, [&](const int r, const int cc, const int j, auto own) {
-                const double zj = z[j];
                 if constexpr (decltype(fresh)::value) {
-                    L.a[r][cc] = stored<MX>(zj);
+                    L.a[r][cc] = static_cast<double>(pn[j]);
                 } else {
                     const double pv = static_cast<double>(po[j]);
                     if constexpr (decltype(own)::value)
                         if (r >= 1 && r <= TY) pc[r - 1] = pv;
-                    L.a[r][cc] = stored<MX>(zj + beta * pv);
+                    L.a[r][cc] = stored<MX>(z[j] + beta * pv);
                 }
                 L.b[r][cc] = K[j];
             });
         };
-        const bool fresh = S->restart != 0;
-        if (fresh) stage(std::true_type{});
-        else stage(std::false_type{});
-        __syncthreads();
-        auto body = [&](auto mu, auto lag) {
+        auto body = [&](auto fresh, auto mu, auto lag) {
             for_tile<TY>(P, col, [&](const int i, const int y, const int r, const int cc) {
                 double wf[4];
                 face_weights(P, y, L.b[r][cc], L.b[r][cc - 1], L.b[r][cc + 1], L.b[r - 1][cc], L.b[r + 1][cc], wf);
                 const double c = L.a[r][cc];
                 const double a = apply_vals(ld_once(chi + i), dt, wf, c, L.a[r][cc - 1], L.a[r][cc + 1],
                                             L.a[r - 1][cc], L.a[r + 1][cc]);
-                st_once(pn + i, static_cast<T>(c));
+                if constexpr (!decltype(fresh)::value) st_once(pn + i, static_cast<T>(c));
                 st_once(Ap + i, a);
                 if constexpr (decltype(lag)::value) {
                     // x + alpha p_old (cg_update's expression), p_old as stored
-                    const double pv = fresh ? static_cast<double>(po[i]) : pc[r - 1];
+                    const double pv = decltype(fresh)::value ? static_cast<double>(po[i]) : pc[r - 1];
                     const double b = decltype(mu)::value ? mu_old[i] - P.mu_ref : ld_once(x + i);
                     st_once(x + i, b + alpha * pv);
                 }
                 v[0] += c * a;
             });
         };
-        with_x_form(S->x_mu, S->x_lag, body);
+        auto go = [&](auto fresh) {
+            stage(fresh);
+            __syncthreads();
+            with_x_form(S->x_mu, S->x_lag, [&](auto mu, auto lag) { body(fresh, mu, lag); });
+        };
+        if (S->restart != 0) go(std::true_type{});
+        else go(std::false_type{});
         block_partial<1, 0u>(v, B.part + P.poff, P.nb, w.lb);
     } else if (state == CG_TRUE) {
         record(B, slot, w.lb, 2);
@@ -1401,7 +1404,7 @@ __global__ __launch_bounds__(TPB) void k_cg_a(Bufs B, int slot) {
                                              L.a[rw - 1][cc], L.a[rw + 1][cc]);
             const double zz = stored<MX>(rr / static_cast<double>(ld_once(diag + i)));
             st_once(r + i, rr);
-            st_once(z + i, static_cast<T>(zz));
+            st_once(pn + i, static_cast<T>(zz));  // the restart's direction p = z (:1062)
             v[0] += rr * rr;
             v[1] += rr * zz;
         });
