@@ -562,9 +562,13 @@ struct Where {
 // parameters are read through the scalar cache; kernels copy their KP by
 // value, which keeps the fields in registers instead of re-reading them
 // after every global store (the stores could alias the parameter block).
+// A single-domain context skips the table: its one domain starts at block 0,
+// so the block's controller state is the first load it waits for (a
+// predicated-off launch exits one load latency sooner).
 __device__ __forceinline__ Where where(const Bufs& B, int only_d) {
     const int g = static_cast<int>(blockIdx.x);
     if (only_d >= 0) return {only_d, g};
+    if (B.ndom == 1) return {0, g};
     const int d = __builtin_amdgcn_readfirstlane(B.blk2dom[g]);
     return {d, g - B.kp[d].boff};
 }

@@ -138,6 +138,43 @@ def test_true_residual_check_failure(pkg, O, name, rtol, k):
         assert o["it"] == int(pd["cg_max"])
 
 
+@pytest.mark.parametrize("name,atol,iters", [("small_48x40", 1e30, 0), ("small_48x40", 1e-4, 1),
+                                             ("small_48x40", 1e-6, 2), ("colony_256x128", 1e30, 0),
+                                             ("colony_256x128", 1e-4, 1), ("colony_256x128", 1e-6, 2)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_short_solves(pkg, O, name, atol, iters, precision):
+    """Solves of 0, 1 and 2 PCG iterations (cg_atol large; closure_tol 1 so
+    the short solve is accepted): finalize reads the solution as the last
+    update left it -- mu_old - mu_ref before the x plane is first written,
+    with or without the last step alpha p still to apply.  FP64 bitwise the
+    oracle in the same blocked order, with the iteration counts the oracle
+    takes; the mixed form takes the same number of iterations, and its
+    iterate differs from the FP64 one by the FP32 rounding of its directions
+    (2^-24 relative) times the steps taken: bar 1e-6 of the solve's largest
+    change of mu (exactly equal after zero iterations)."""
+    g = load_golden(name)
+    pd = dict(g["pd"])
+    pd["cg_atol"], pd["closure_tol"] = atol, 1.0
+    say("short solve", name, atol, iters, precision)
+    prec = pkg.FP64 if precision == "fp64" else pkg.MIXED
+    with pkg.Solver(pd, float(g["sc0"][9]), precision=prec) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        k = s.layout()[1]
+        d = s.implicit_diffusion(g["propose_phi"], float(g["dt_probe"]))
+    P = O.params_from_dict(pd, red_mode=O.RED_BLOCKED, red_k=k)
+    o = O.implicit_diffusion(P, g["propose_phi"], g["c0"], g["mu0"], float(g["dt_probe"]), float(g["sc0"][9]))
+    assert o["it"] == iters and o["ok"] == 1
+    assert (d["ok"], d["it"]) == (o["ok"], o["it"])
+    if precision == "fp64":
+        assert d["residual"] == o["residual"] and d["closure"] == o["closure"]
+        assert np.array_equal(d["carbon"], o["carbon"]) and np.array_equal(d["mu"], o["mu"])
+    else:
+        moved = np.abs(o["mu"] - g["mu0"]).max()
+        dmu = np.abs(d["mu"] - o["mu"]).max()
+        say("  mixed: max|dmu|", dmu, "of a change of", moved)
+        assert dmu <= 1e-6 * moved if iters else dmu == 0.0
+
+
 def test_true_residual_check_failure_in_steps(pkg, O):
     """Whole accepted steps whose solves restart after a failed true-residual
     check (cg_rtol 1e-16, cg_atol 0): step infos, fields and scalars bitwise
