@@ -109,7 +109,6 @@ struct DevScal {
     // step (p in buffer pidx), which the next kernel that needs x applies.
     int x_mu, x_lag;
     int err;               // ERRF_* bits
-    unsigned counter;      // last-block counter
     double change;
     double rhs2, rz, rr, pAp, alpha, beta, tol, cg_res;
     double closure, valid, mass, energy;
@@ -1283,16 +1282,6 @@ __device__ __forceinline__ void with_x_form(int mu, int lag, F&& f) {
         else f(std::false_type{}, std::false_type{});
     }
 }
-
-// Search direction and diffusivity of one cell.
-struct DirK {
-    double p, k;
-};
-
-// Solution and diffusivity of one cell.
-struct SolK {
-    double x, k;
-};
 
 // PCG kernel A.  In state CG_ITER: the search direction of this iteration
 // p = z + beta p_old (:1079), kept in the other p buffer (after a
