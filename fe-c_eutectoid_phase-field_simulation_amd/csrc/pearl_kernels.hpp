@@ -266,7 +266,7 @@ __device__ __forceinline__ void block_partial(double (&v)[NQ], double* __restric
 // one dependent load per partial.
 template <int NQ, unsigned OPS>
 __device__ __forceinline__ bool final_reduce(double (&v)[NQ], const double* __restrict__ part, int nb) {
-    constexpr int FOLD = NQ <= 2 ? 32 : (NQ <= 4 ? 16 : 8);
+    constexpr int FOLD = NQ <= 4 ? 32 : 16;
     __shared__ double sh[4][NQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
