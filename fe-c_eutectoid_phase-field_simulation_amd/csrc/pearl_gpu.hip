@@ -88,9 +88,12 @@ struct pearl_ctx {
     int cg_slots = 6;  // PCG slots per round
     // graphs of one round: [0] plain, [1] with the state-reduction kernels,
     // [2] with the fixed-run state copy (adjoint segment recompute)
-    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
-    hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
-    int graph_slots[3] = {0, 0, 0};
+    // captured rounds: [0] plain, [1] plain after a leading begin, [2] with
+    // the state reductions after a leading begin, [3] ending in the fixed-run
+    // state copy, [4] the same after a leading begin
+    hipGraph_t graph[5] = {};
+    hipGraphExec_t graph_exec[5] = {};
+    int graph_slots[5] = {};
     bool any_phase_only = false;  // a phase-only domain: rounds carry its kernels
     // profiling
     bool prof = false;
@@ -203,17 +206,19 @@ hipEvent_t get_event(pearl_ctx* c) {
     return e;
 }
 
-// Enqueue one round: begin, state, propose, prepare, slots x
-// (cg_a, cg_b), finalize, phase-only, decide.  With `timed`, each cell
+// Enqueue one round: [begin], [state], propose, prepare, slots x
+// (cg_a, cg_b), finalize, [phase-only], [turn], [fixed-run copy].  `turn`
+// decides the attempt and starts the next step or attempt (k_turn), so the
+// round after it needs no begin: only the first round of a batch carries a
+// leading begin, for the state the host left.  With `timed`, each cell
 // kernel and its finisher are bracketed by events (the cell kernel alone,
 // then the finisher) and the cell kernel writes its work record into
-// kinds[slot].
-// One round of kernels.  The state reductions (e0, mass0 of a step) run only
-// in a round whose flag asks for them: their cached values only turn valid
+// kinds[slot].  The state reductions (e0, mass0 of a step) run only in a
+// round whose flag asks for them: their cached values only turn valid
 // during a run (host calls invalidate them), so a run needs them at most in
 // its first round.  The phase-only kernels are present only when a domain of
 // the context is phase-only.  A predicated-off launch costs several us.
-void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_decide, bool with_state,
+void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool with_turn, bool with_state,
                    bool with_save = false) {
     const dim3 g(static_cast<unsigned>(c->total_blocks));
     const dim3 gd((c->ndom + 63) / 64);
@@ -258,7 +263,7 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     if (c->any_phase_only)
         ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
            [&] { hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B); });
-    if (with_decide) hipLaunchKernelGGL(k_decide, gd, dim3(64), 0, st, B);
+    if (with_turn) hipLaunchKernelGGL(k_turn, gd, dim3(64), 0, st, B);
     if (with_save) {
         hipLaunchKernelGGL(k_fixed_save, dim3(static_cast<unsigned>(std::min<long long>(c->total_blocks, 2048))), tb, 0,
                            st, B);
@@ -330,7 +335,7 @@ int run_rounds(pearl_ctx* c, int rounds, bool state_first, bool save = false) {
     if (c->prof) {
         for (int r = 0; r < rounds; ++r) {
             hipLaunchKernelGGL(k_clear_kinds, dim3(1), dim3(MAX_SLOTS), 0, c->stream, c->d_kinds, MAX_SLOTS);
-            enqueue_round(c, c->cg_slots, true, true, true, state_first && r == 0, save);
+            enqueue_round(c, c->cg_slots, true, r == 0, true, state_first && r == 0, save);
             if (r + 1 < rounds) {
                 const int st = read_scalars(c);
                 if (st) return st;
@@ -338,7 +343,7 @@ int run_rounds(pearl_ctx* c, int rounds, bool state_first, bool save = false) {
         }
         return PEARL_OK;
     }
-    const int first = save ? 2 : (state_first ? 1 : 0), rest = save ? 2 : 0;
+    const int first = save ? 4 : (state_first ? 2 : 1), rest = save ? 3 : 0;
     for (int v : {first, rest}) {
         if (c->graph_exec[v] && c->graph_slots[v] == c->cg_slots) continue;
         if (c->graph_exec[v]) (void)hipGraphExecDestroy(c->graph_exec[v]);
@@ -346,7 +351,7 @@ int run_rounds(pearl_ctx* c, int rounds, bool state_first, bool save = false) {
         c->graph_exec[v] = nullptr;
         c->graph[v] = nullptr;
         HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_round(c, c->cg_slots, false, true, true, v == 1, v == 2);
+        enqueue_round(c, c->cg_slots, false, v == 1 || v == 2 || v == 4, true, v == 2, v == 3 || v == 4);
         HIP_CHECK(c, hipStreamEndCapture(c->stream, &c->graph[v]));
         HIP_CHECK(c, hipGraphInstantiate(&c->graph_exec[v], c->graph[v], nullptr, nullptr, 0));
         c->graph_slots[v] = c->cg_slots;
@@ -733,7 +738,7 @@ int pearl_destroy(pearl_ctx* c) {
         (void)hipEventDestroy(e.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 5; ++v) {
         if (c->graph_exec[v]) (void)hipGraphExecDestroy(c->graph_exec[v]);
         if (c->graph[v]) (void)hipGraphDestroy(c->graph[v]);
     }

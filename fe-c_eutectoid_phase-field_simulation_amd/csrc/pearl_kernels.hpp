@@ -11,10 +11,10 @@
 //
 // Control.  advance() (:1119-1174) runs on the device as a small state
 // machine per domain (DevScal::phase).  The host enqueues fixed "rounds" of
-// kernels -- begin, state reductions, propose, prepare (setup + initial
-// residual), PCG slots, finalize, decide -- and every kernel acts only on the
-// domains whose phase
-// and PCG state call for it.  A round therefore has no host decision in it:
+// kernels -- state reductions, propose, prepare (setup + initial residual),
+// PCG slots, finalize, then the decision and the start of the next attempt
+// (k_turn; a batch's first round also starts with k_begin) -- and every
+// kernel acts only on the domains whose phase and PCG state call for it.  A round therefore has no host decision in it:
 // it can be replayed from a captured graph, and one launch serves any number
 // of independent domains of different sizes.
 #pragma once
@@ -682,11 +682,9 @@ __device__ __forceinline__ void cg_head(const KP& P, DevScal* S, int it) {
 // Round kernels
 // ---------------------------------------------------------------------------
 
-// Start of a step (:1120-1123) and reset of the per-attempt flags.  One
-// thread per domain.
-__global__ void k_begin(Bufs B) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= B.ndom) return;
+// Start of a step (:1120-1123) and reset of the per-attempt flags, for
+// domain d.
+__device__ __forceinline__ void begin_domain(const Bufs& B, int d) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
     if (S->phase == PH_START) {
@@ -720,6 +718,12 @@ __global__ void k_begin(Bufs B) {
         S->pidx = 0;
         S->pad = 0;  // "stepped": k_cg_a took a step that k_cg_b must complete
     }
+}
+
+// One thread per domain.
+__global__ void k_begin(Bufs B) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < B.ndom) begin_domain(B, d);
 }
 
 // valid_state / total_carbon / reduced_energy of one cell of a state
@@ -1734,11 +1738,9 @@ __global__ __launch_bounds__(TPB) void k_phase_only_fin(Bufs B) {
     S->phase = PH_DECIDE;
 }
 
-// Acceptance tests and the dt controller of advance (:1131-1173).  One
-// thread per domain.
-__global__ void k_decide(Bufs B) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= B.ndom) return;
+// Acceptance tests and the dt controller of advance (:1131-1173), for
+// domain d.
+__device__ __forceinline__ void decide_domain(const Bufs& B, int d) {
     const KP& P = B.kp[d];
     DevScal* S = B.S + d;
     if (S->phase != PH_DECIDE) return;
@@ -1832,6 +1834,16 @@ __global__ void k_decide(Bufs B) {
         S->attempt++;
         S->phase = PH_PROPOSE;
     }
+}
+
+// The end of a round: the decision on the attempt, then the start of the
+// next step or attempt, in one launch (each launch costs ~4.5 us however
+// little it does).  One thread per domain.
+__global__ void k_turn(Bufs B) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= B.ndom) return;
+    decide_domain(B, d);
+    begin_domain(B, d);
 }
 
 // Fixed-dt runs that keep every state (the adjoint's segment recompute):
