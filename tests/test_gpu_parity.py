@@ -483,3 +483,49 @@ def test_profiled_and_graph_runs_agree(pkg):
                 assert st["propose_phi"]["launches"] >= 15 and st["cg_apply"]["bytes"] > 0
     for a, b in zip(out[0][:3], out[1][:3]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_with_restarting_domain(pkg, O, precision):
+    """Domains that share one set of launches while their solves take
+    different paths: the first domain's solves fail the true-residual check
+    inside finalize and restart (cg_rtol 1e-16, cg_atol 0), the second solves
+    normally, the third converges in one iteration (cg_atol 1e-4,
+    closure_tol 1).  Each domain bitwise the same domain alone in its own
+    context; the FP64 batch also bitwise the oracle in the blocked order."""
+    names = ["small_48x40", "colony_256x128", "window_alpha_32x40"]
+    gs = [load_golden(n) for n in names]
+    pds = [dict(g["pd"]) for g in gs]
+    pds[0]["cg_rtol"], pds[0]["cg_atol"] = 1e-16, 0.0
+    pds[2]["cg_atol"], pds[2]["closure_tol"] = 1e-4, 1.0
+    steps = [4, 6, 3]
+    prec = pkg.FP64 if precision == "fp64" else pkg.MIXED
+    say("batch with a restarting domain", precision)
+    with pkg.Batch(pds, [float(g["sc0"][9]) for g in gs], precision=prec) as b:
+        k = b.layout()[1]
+        for d, g in enumerate(gs):
+            b.set_domain_state(d, g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        r = b.run_all([float(g["end_time"]) for g in gs], steps)
+        assert r["rc"] == 0, r
+        assert r["taken"] == steps
+        passes = b.kernel_stats()["true_residual"]["launches"]
+        states = [b.get_domain_state(d) for d in range(len(gs))]
+    assert passes >= steps[0]
+    for d, g in enumerate(gs):
+        with pkg.Solver(pds[d], float(g["sc0"][9]), cells_per_thread=k, precision=prec) as s:
+            s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+            for _ in range(steps[d]):
+                s.advance(float(g["end_time"]))
+            alone = s.get_state()
+        for a, bb in zip(states[d][:3], alone[:3]):
+            assert np.array_equal(a, bb), names[d]
+        assert states[d][3].to_list() == alone[3].to_list(), names[d]
+        if precision == "fp64":
+            P = O.params_from_dict(pds[d], red_mode=O.RED_BLOCKED, red_k=k)
+            S = O.OracleState(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+            for _ in range(steps[d]):
+                rc, _ = O.advance(P, S, float(g["sc0"][9]), float(g["end_time"]))
+                assert rc == 0
+            phi, c, mu, sc = states[d]
+            assert np.array_equal(phi, S.phi) and np.array_equal(c, S.carbon) and np.array_equal(mu, S.mu), names[d]
+            assert sc.to_list() == S.scalars(), names[d]
