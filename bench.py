@@ -822,8 +822,11 @@ def main():
         copy_gbs = pkg.stream_probe(0, 1 << 30, 20)
         log(f"copy ceiling: {copy_gbs:.1f} GB/s (1 GiB device copy, pearl_stream_probe)")
         if args.precision == "fp64":
-            # each kernel's own access pattern without its arithmetic
+            # each kernel's own access pattern without its arithmetic (for
+            # the kernels that moved bytes in the region)
             for name, (r, w) in pkg.KERNEL_STREAMS.items():
+                if not stats.get(name, {}).get("bytes"):
+                    continue
                 mix[name] = pkg.stream_mix(r, w, nx, ny, s.layout()[1])
                 log(f"  bare pattern {name:16s} {r} read + {w} written planes: {mix[name][0]:7.1f} us, "
                     f"{mix[name][1]:7.1f} GB/s")
