@@ -175,15 +175,18 @@ def test_short_solves(pkg, O, name, atol, iters, precision):
         assert dmu <= 1e-6 * moved if iters else dmu == 0.0
 
 
-def test_true_residual_check_failure_in_steps(pkg, O):
+@pytest.mark.parametrize("cg_max", [2000, 9, 8])
+def test_true_residual_check_failure_in_steps(pkg, O, cg_max):
     """Whole accepted steps whose solves restart after a failed true-residual
-    check (cg_rtol 1e-16, cg_atol 0): step infos, fields and scalars bitwise
-    the oracle's."""
+    check (cg_rtol 1e-16, cg_atol 0); with cg_max 9 and 8 some attempts end
+    unconverged, a restart on their last allowed iteration among them, and
+    are retried with half the step: step infos, fields and scalars bitwise the
+    oracle's."""
     g = load_golden("small_48x40")
     pd = dict(g["pd"])
-    pd["cg_rtol"], pd["cg_atol"] = 1e-16, 0.0
+    pd["cg_rtol"], pd["cg_atol"], pd["cg_max"] = 1e-16, 0.0, cg_max
     end, steps = float(g["end_time"]), 4
-    say("true residual check failure in steps")
+    say("true residual check failure in steps, cg_max", cg_max)
     with pkg.Solver(pd, float(g["sc0"][9])) as s:
         s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
         k = s.layout()[1]

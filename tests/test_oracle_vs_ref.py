@@ -44,6 +44,36 @@ def test_trajectories_bitwise(uc, pairs, nx, ny):
     assert np.array_equal(phi1, S.phi) and np.array_equal(c1, S.carbon) and np.array_equal(mu1, S.mu)
 
 
+@pytest.mark.parametrize("cg_max", [2000, 9, 8])
+def test_restarts_and_iteration_limit_bitwise(cg_max):
+    """Solves whose recursive residual passes while the true residual does
+    not (cg_rtol 1e-16, cg_atol at the input's smallest value), so they
+    restart from the true residual; with cg_max 9 and 8 some end unconverged
+    and the step is retried.  The oracle's advance equals the reference's,
+    step info and fields."""
+    I = inputs_mod()
+    p = I.colony_input(tempfile.mktemp(suffix=".in"), 40, 24, 2, 8, 6e-10,
+                       overrides={"smooth_steps": 10, "smooth_min_steps": 10, "cg_rtol": "1e-16",
+                                  "cg_atol": "1e-300", "cg_max": str(cg_max)})
+    R = O.Ref(p)
+    R.initialize()
+    phi, c, mu, sc = R.get_state()
+    P = O.params_from_dict(R.pd)
+    S = O.OracleState(phi, c, mu, sc[:9])
+    retries = 0
+    for _ in range(12):
+        info, err = R.advance(6e-10)
+        rc, oi = O.advance(P, S, sc[9], 6e-10)
+        assert not err and rc == 0
+        assert list(info[:7]) == [oi.accepted_dt, oi.max_phase_change, oi.linear_residual, oi.closure,
+                                  oi.energy_change, oi.cg_iterations, oi.retries]
+        retries += oi.retries
+    if cg_max < 10:
+        assert retries > 0
+    phi1, c1, mu1, _ = R.get_state()
+    assert np.array_equal(phi1, S.phi) and np.array_equal(c1, S.carbon) and np.array_equal(mu1, S.mu)
+
+
 def test_smoothing_bitwise():
     I = inputs_mod()
     p = I.colony_input(tempfile.mktemp(suffix=".in"), 48, 32, 2, 10, 1e-9)
