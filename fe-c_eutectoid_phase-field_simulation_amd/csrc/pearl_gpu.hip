@@ -239,6 +239,20 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
         }
         ++slot;
     };
+    // a cell kernel whose reduction needs no finisher (propose: atomic maxima
+    // that the next kernel reads)
+    auto ev_cell = [&](int kid, auto&& cell) {
+        if (timed) {
+            hipEvent_t a = get_event(c), b = get_event(c);
+            (void)hipEventRecord(a, c->stream);
+            cell(slot);
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({kid, slot, a, b});
+        } else {
+            cell(-1);
+        }
+        ++slot;
+    };
     // finisher kernels: one block per domain
     const dim3 gf(static_cast<unsigned>(c->ndom));
     const dim3 tb(TPB);
@@ -248,8 +262,7 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
     if (with_state)
         ev(PEARL_KERNEL_STATE_REDUCE, [&](int s) { PEARL_LAUNCH(c, k_state, g, B, s); },
            [&] { hipLaunchKernelGGL(k_state_fin, gf, tb, 0, st, B); });
-    ev(PEARL_KERNEL_PROPOSE, [&](int s) { launch_propose(c, g, B, s); },
-       [&] { hipLaunchKernelGGL(k_propose_fin, gf, tb, 0, st, B); });
+    ev_cell(PEARL_KERNEL_PROPOSE, [&](int s) { launch_propose(c, g, B, s); });
     ev(PEARL_KERNEL_PREPARE, [&](int s) { PEARL_LAUNCH_P(c, k_prepare, g, B, s); },
        [&] { hipLaunchKernelGGL(k_prepare_fin, gf, tb, 0, st, B); });
     for (int j = 0; j < slots; ++j) {
@@ -998,6 +1011,8 @@ int pearl_propose_phi(pearl_ctx* c, double dt, double* phi_out, double* max_chan
         h.need_state = 0;
         h.dt = dt;
         h.status = ST_OK;
+        for (auto& b : h.chg) b = 0;  // propose's reduction slots (the device's begin clears them in a round)
+        h.proj = 0;
         st = write_scalars(c);
         if (st) return st;
         launch_propose(c, dim3(static_cast<unsigned>(c->total_blocks)), c->B, -1);
@@ -1033,7 +1048,7 @@ int pearl_implicit_diffusion(pearl_ctx* c, const double* phi_trial, double dt, d
         DevScal& h = c->hS[0];
         const size_t n = static_cast<size_t>(c->kp[0].n);
         HIP_CHECK(c, hipMemcpyAsync(dom_phi(c, 0, h.cur ^ 1), phi_trial, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        h.phase = PH_SOLVE;
+        h.phase = PH_PREPARE;
         h.live = 1;
         h.cg_alone = 0;
         h.cg_state = CG_IDLE;

@@ -67,8 +67,9 @@ struct KP {
     int cg_max, max_retries, check_energy, phase_only;
 };
 
-// Controller phases of one domain.
-enum { PH_IDLE = 0, PH_START = 1, PH_PROPOSE = 2, PH_SOLVE = 3, PH_DECIDE = 4 };
+// Controller phases of one domain.  PH_PREPARE: the trial phases are given
+// (the single implicit_diffusion entry) and the solve starts at prepare.
+enum { PH_IDLE = 0, PH_START = 1, PH_PROPOSE = 2, PH_SOLVE = 3, PH_DECIDE = 4, PH_PREPARE = 5 };
 // PCG states (implicit_diffusion :1047-1080).  CG_ITER: take a step;
 // CG_TRUE: recompute the true residual (:1053) in a pass of its own;
 // CG_CHECK: the recursive residual passed, and the finalize pass computes
@@ -104,6 +105,12 @@ struct DevScal {
     int live;              // the phase increment passed (:1131)
     int cg_state, cg_it, restart, pidx;
     int cg_alone;          // a solve with no finalize pass after it (the adjoint's): the true residual is a pass of its own
+    // propose_phi's reduction: the bits of the largest phase change (a
+    // non-negative double orders like its bits), folded by atomicMax into
+    // one of 8 slots per block (spread over the XCDs), and the projection
+    // failure flag; cleared at the start of every attempt
+    unsigned long long chg[8];
+    int proj;
     // The forward solve's x = mu - mu_ref: x_mu, the plane is not written yet
     // and x is mu_old - mu_ref; x_lag, x still lacks alpha p of the last
     // step (p in buffer pidx), which the next kernel that needs x applies.
@@ -708,6 +715,9 @@ __device__ __forceinline__ void begin_domain(const Bufs& B, int d) {
             (!S->fixed && ((P.check_energy && !S->e0_valid) || (!P.phase_only && !S->m0_valid))) ? 1 : 0;
     }
     if (S->phase == PH_PROPOSE) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) S->chg[k] = 0;
+        S->proj = 0;
         S->live = 0;
         S->cg_alone = 0;
         S->cg_state = CG_IDLE;
@@ -1093,27 +1103,62 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
             }
         }
     }
-    block_partial<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb, w.lb);
+    // the block's largest change and projection flag (the same tree as the
+    // blocked reduction; a maximum is exact in any order), then one atomic
+    // per block: no partials and no finisher launch
+    {
+        __shared__ double sh[4][2];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        wave_tree<2, (1u | (1u << 2))>(v);
+        if (lane == 0) {
+            sh[wv][0] = v[0];
+            sh[wv][1] = v[1];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double m = dmaxd(dmaxd(sh[0][0], sh[1][0]), dmaxd(sh[2][0], sh[3][0]));
+            atomicMax(&S->chg[blockIdx.x & 7], static_cast<unsigned long long>(__double_as_longlong(m)));
+            if (dmaxd(dmaxd(sh[0][1], sh[1][1]), dmaxd(sh[2][1], sh[3][1])) > 0) atomicOr(&S->proj, 1);
+        }
+    }
 }
 
-__global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
-    const int d = blockIdx.x;
-    const KP P = B.kp[d];
-    DevScal* S = B.S + d;
-    if (S->phase != PH_PROPOSE || S->need_state) return;
-    double v[2];
-    if (!final_reduce<2, (1u | (1u << 2))>(v, B.part + P.poff, P.nb)) return;
+// propose_phi's result for a domain (after the propose launch): the
+// largest phase change and whether the attempt goes on (:1131; a fixed-dt
+// step always continues); 2 = a projection failed.
+__device__ __forceinline__ int propose_verdict(const KP& P, const DevScal* S, double& change) {
+    unsigned long long b = S->chg[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) b = S->chg[k] > b ? S->chg[k] : b;
+    change = __longlong_as_double(static_cast<long long>(b));
+    if (S->proj) return 2;
+    return (S->fixed || (__builtin_isfinite(change) && change <= P.max_phase_change)) ? 0 : 1;
+}
+
+// propose_phi's bookkeeping (the one thread that writes the controller):
+// the change for StepInfo, then a projection failure ends the step, a
+// failed increment test decides the attempt, and otherwise the solve runs.
+__device__ __forceinline__ void propose_record(DevScal* S, int verdict, double change) {
     S->runs[RUN_PROPOSE]++;
-    S->change = v[0];
-    if (v[1] > 0) {
+    S->change = change;
+    if (verdict == 2) {
         S->status = ST_PROJECTION;
         S->phase = PH_IDLE;
         return;
     }
-    // :1131 -- the rest of the attempt runs only if the increment passes
-    // (a fixed-dt step always continues)
-    S->live = (S->fixed || (__builtin_isfinite(v[0]) && v[0] <= P.max_phase_change)) ? 1 : 0;
-    S->phase = S->live ? PH_SOLVE : PH_DECIDE;
+    S->live = verdict == 0 ? 1 : 0;
+    S->phase = verdict == 0 ? PH_SOLVE : PH_DECIDE;
+}
+
+// The propose launch's result on its own (the single propose_phi entry; in
+// a round prepare's finisher records it).
+__global__ __launch_bounds__(TPB) void k_propose_fin(Bufs B) {
+    const int d = blockIdx.x;
+    const KP P = B.kp[d];
+    DevScal* S = B.S + d;
+    if (S->phase != PH_PROPOSE || S->need_state || threadIdx.x != 0) return;
+    double change;
+    propose_record(S, propose_verdict(P, S, change), change);
 }
 
 // Storage type of the Jacobi diagonal and the PCG vectors p and z: double in
@@ -1162,7 +1207,12 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     const Where w = where(B, -1);
     const KP P = B.kp[w.d];
     DevScal* S = B.S + w.d;
-    if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
+    // right after this round's propose launch: every block reads its
+    // verdict, and the solve starts only if the attempt goes on
+    const bool given = S->phase == PH_PREPARE;
+    if (!(given || (S->phase == PH_PROPOSE && !S->need_state)) || P.phase_only) return;
+    double change;
+    if (!given && propose_verdict(P, S, change) != 0) return;
     record(B, slot, w.lb, 1);
     const int n = P.n;
     const int cur = S->cur;
@@ -1239,13 +1289,24 @@ __global__ __launch_bounds__(TPB) void k_prepare(Bufs B, int slot) {
     block_partial<4, (1u << 6)>(v, B.part + P.poff, P.nb, w.lb);
 }
 
+// Records propose's result (propose_record) and, when the solve runs,
+// finishes prepare's sums.
 __global__ __launch_bounds__(TPB) void k_prepare_fin(Bufs B) {
     const int d = blockIdx.x;
     const KP P = B.kp[d];
     DevScal* S = B.S + d;
-    if (!(S->phase == PH_SOLVE && S->cg_state == CG_IDLE) || P.phase_only) return;
+    const bool given = S->phase == PH_PREPARE;
+    if (!(given || (S->phase == PH_PROPOSE && !S->need_state))) return;
+    double change = 0;
+    const int verdict = given ? 0 : propose_verdict(P, S, change);
+    if (verdict != 0 || P.phase_only) {
+        if (threadIdx.x == 0) propose_record(S, verdict, change);
+        return;
+    }
     double v[4];
     if (!final_reduce<4, (1u << 6)>(v, B.part + P.poff, P.nb)) return;
+    if (given) S->phase = PH_SOLVE;
+    else propose_record(S, verdict, change);
     S->runs[RUN_PREPARE]++;
     if (v[3] > 0) {
         S->status = ST_MIXTURE;
