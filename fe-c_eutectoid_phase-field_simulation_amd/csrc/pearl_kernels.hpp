@@ -1117,7 +1117,11 @@ __global__ __launch_bounds__(TPB) void k_propose(Bufs B, int slot) {
         __syncthreads();
         if (threadIdx.x == 0) {
             const double m = dmaxd(dmaxd(sh[0][0], sh[1][0]), dmaxd(sh[2][0], sh[3][0]));
-            atomicMax(&S->chg[blockIdx.x & 7], static_cast<unsigned long long>(__double_as_longlong(m)));
+            // a zero maximum changes no slot, so a block of unchanged cells
+            // (a colony's bulk) issues no atomic: same-address atomics from
+            // every block, arriving together, serialize past the XCDs' L2s
+            const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(m));
+            if (bits != 0) atomicMax(&S->chg[blockIdx.x & 7], bits);
             if (dmaxd(dmaxd(sh[0][1], sh[1][1]), dmaxd(sh[2][1], sh[3][1])) > 0) atomicOr(&S->proj, 1);
         }
     }
