@@ -81,6 +81,80 @@ def test_propose_phi_out_of_range_numerators(pkg, O, k):
     assert np.array_equal(np.signbit(got.reshape(want.shape)), np.signbit(want))
 
 
+@pytest.mark.parametrize("k", [1, 4])
+def test_propose_phi_change_reduction_edges(pkg, O, k):
+    """propose_phi's largest change is folded by per-block atomic maxima on
+    its bits, and a block whose maximum is +0 takes no atomic.  Edges of that
+    reduction, each bitwise the oracle's sequential propose_phi:
+     - every cell bulk austenite: no block changes, so no atomic is taken,
+       and the change is exactly 0 with the phases unchanged;
+     - one changed cell in the domain's last tile: a single block carries
+       the maximum;
+     - a chemical potential whose psi overflows in one cell: that cell's
+       trial value is not finite, the change is +inf and the cell keeps its
+       phases."""
+    g = load_golden("small_48x40")
+    ny, nx = g["mu0"].shape
+    mu = np.asarray(g["mu0"], dtype=np.float64)
+    dt = float(g["dt_probe"])
+    P = O.params_from_dict(g["pd"])
+    bulk = np.zeros((3, ny, nx))
+    bulk[2] = 1.0
+    one = bulk.copy()
+    one[:, ny - 1, nx - 2] = (0.25, 0.125, 0.625)
+    huge = mu.copy()
+    huge[ny // 2, 5] = 1e300
+    cases = [("bulk", bulk, mu, 0.0), ("one cell in the last tile", one, mu, None),
+             ("psi overflow in one cell", np.asarray(g["phi0"], dtype=np.float64), huge, np.inf)]
+    for label, phi, m, expect in cases:
+        say("propose_phi reduction edge:", label, "k", k)
+        want_ch, want, st = O.propose_phi(P, phi, m, dt)
+        assert st == 0
+        if expect is not None:
+            assert want_ch == expect
+        with pkg.Solver(g["pd"], float(g["sc0"][9]), cells_per_thread=k) as s:
+            s.set_state(phi, g["c0"], m, g["sc0"][:9])
+            ch, got = s.propose_phi(dt)
+        assert ch == want_ch
+        assert np.array_equal(got.reshape(want.shape), want)
+        if label == "bulk":
+            assert np.array_equal(got.reshape(phi.shape), phi)
+
+
+def test_projection_failure(pkg, O):
+    """A step so large that a trial value reaches 2^53: the projection finds
+    no active set (pearl_serial.cpp:505 throws).  The oracle reports it
+    (status 5, ORC_ERR_PROJECTION), the single propose entry returns
+    PEARL_ERR_PROJECTION, and a whole step started by advance ends with the
+    same error and leaves the state as it was."""
+    g = load_golden("small_48x40")
+    dt = float(g["dt_probe"]) * 1e20
+    P = O.params_from_dict(g["pd"])
+    _, _, st = O.propose_phi(P, np.asarray(g["phi0"], dtype=np.float64), np.asarray(g["mu0"], dtype=np.float64), dt)
+    assert st == 5
+    say("projection failure: single entry")
+    with make(pkg, g) as s:
+        with pytest.raises(pkg.PearlError) as e:
+            s.propose_phi(dt)
+        assert e.value.code == pkg.ERR_PROJECTION
+    say("projection failure: advance")
+    pd = dict(g["pd"])
+    pd["dt_max"] = 2 * dt
+    with pkg.Solver(pd, float(g["sc0"][9])) as s:
+        s.set_state(g["phi0"], g["c0"], g["mu0"], g["sc0"][:9])
+        sc = s.scalars()
+        sc.dt = dt
+        s.set_scalars(sc)
+        before = s.get_state()
+        with pytest.raises(pkg.PearlError) as e:
+            s.advance(10 * dt)
+        assert e.value.code == pkg.ERR_PROJECTION
+        after = s.get_state()
+        for x, y in zip(before[:3], after[:3]):
+            assert np.array_equal(x, y)
+        assert after[3].step == before[3].step and after[3].time == before[3].time
+
+
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("k", [1, 2, 4, 8, 16])
 def test_implicit_diffusion(pkg, O, name, k):
