@@ -208,8 +208,8 @@ hipEvent_t get_event(pearl_ctx* c) {
 
 // Enqueue one round: [begin], [state], propose, prepare, slots x
 // (cg_a, cg_b), finalize, [phase-only], [turn], [fixed-run copy].  `turn`
-// decides the attempt and starts the next step or attempt (k_turn), so the
-// round after it needs no begin: only the first round of a batch carries a
+// decides the attempt and starts the next step or attempt (taken by the
+// round's last finisher, turn_domain), so the round after it needs no begin: only the first round of a batch carries a
 // leading begin, for the state the host left.  With `timed`, each cell
 // kernel and its finisher are bracketed by events (the cell kernel alone,
 // then the finisher) and the cell kernel writes its work record into
@@ -271,12 +271,20 @@ void enqueue_round(pearl_ctx* c, int slots, bool timed, bool with_begin, bool wi
         ev(PEARL_KERNEL_CG_UPDATE, [&](int s) { launch_cg_b(c, g, B, s); },
            [&] { hipLaunchKernelGGL(k_cg_b_fin, gf, tb, 0, st, B); });
     }
+    // the round's last finisher also takes the turn when the round ends in
+    // one
+    const bool turn_fin = with_turn && !c->any_phase_only, turn_po = with_turn && c->any_phase_only;
     ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH_P(c, k_finalize, g, B, s); },
-       [&] { hipLaunchKernelGGL(k_finalize_fin, gf, tb, 0, st, B); });
+       [&] {
+           if (turn_fin) hipLaunchKernelGGL(k_finalize_fin<true>, gf, tb, 0, st, B);
+           else hipLaunchKernelGGL(k_finalize_fin<false>, gf, tb, 0, st, B);
+       });
     if (c->any_phase_only)
         ev(PEARL_KERNEL_FINALIZE, [&](int s) { PEARL_LAUNCH(c, k_phase_only, g, B, s); },
-           [&] { hipLaunchKernelGGL(k_phase_only_fin, gf, tb, 0, st, B); });
-    if (with_turn) hipLaunchKernelGGL(k_turn, gd, dim3(64), 0, st, B);
+           [&] {
+               if (turn_po) hipLaunchKernelGGL(k_phase_only_fin<true>, gf, tb, 0, st, B);
+               else hipLaunchKernelGGL(k_phase_only_fin<false>, gf, tb, 0, st, B);
+           });
     if (with_save) {
         hipLaunchKernelGGL(k_fixed_save, dim3(static_cast<unsigned>(std::min<long long>(c->total_blocks, 2048))), tb, 0,
                            st, B);

@@ -13,7 +13,8 @@
 // machine per domain (DevScal::phase).  The host enqueues fixed "rounds" of
 // kernels -- state reductions, propose, prepare (setup + initial residual),
 // PCG slots, finalize, then the decision and the start of the next attempt
-// (k_turn; a batch's first round also starts with k_begin) -- and every
+// (the turn, taken by the round's last finisher; a batch's first round also
+// starts with k_begin) -- and every
 // kernel acts only on the domains whose phase and PCG state call for it.  A round therefore has no host decision in it:
 // it can be replayed from a captured graph, and one launch serves any number
 // of independent domains of different sizes.
@@ -1721,8 +1722,8 @@ __global__ __launch_bounds__(TPB) void k_finalize(Bufs B, int slot) {
     block_partial<5, (1u | (2u << 2))>(v, B.part + P.poff, P.nb, w.lb);
 }
 
-__global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
-    const int d = blockIdx.x;
+// finalize's finisher for domain d (one 256-thread block; k_finalize_fin)
+__device__ __forceinline__ void finalize_fin_domain(const Bufs& B, int d) {
     const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && (S->cg_state == CG_OK || S->cg_state == CG_CHECK))) return;
@@ -1783,8 +1784,8 @@ __global__ __launch_bounds__(TPB) void k_phase_only(Bufs B, int slot) {
     block_partial<5, (1u | (2u << 2) | (1u << 8))>(v, B.part + P.poff, P.nb, w.lb);
 }
 
-__global__ __launch_bounds__(TPB) void k_phase_only_fin(Bufs B) {
-    const int d = blockIdx.x;
+// the phase-only acceptance's finisher for domain d (k_phase_only_fin)
+__device__ __forceinline__ void phase_only_fin_domain(const Bufs& B, int d) {
     const KP P = B.kp[d];
     DevScal* S = B.S + d;
     if (!(S->phase == PH_SOLVE && P.phase_only)) return;
@@ -1901,14 +1902,37 @@ __device__ __forceinline__ void decide_domain(const Bufs& B, int d) {
     }
 }
 
-// The end of a round: the decision on the attempt, then the start of the
-// next step or attempt, in one launch (each launch costs ~4.5 us however
-// little it does).  One thread per domain.
-__global__ void k_turn(Bufs B) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= B.ndom) return;
+// The end of a round for domain d, the turn: the decision on the attempt,
+// then the start of the next step or attempt.
+__device__ __forceinline__ void turn_domain(const Bufs& B, int d) {
     decide_domain(B, d);
     begin_domain(B, d);
+}
+
+// The finishers of the acceptance passes, one block per domain.  TURN: the
+// finisher is the round's last launch and also takes its domain's turn, so
+// the turn needs no launch of its own (each launch costs ~4.5 us however
+// little it does).  Every domain's block takes the turn, whether or not its
+// finisher had work this round.  The barrier orders the turn's writes to the
+// controller after every thread's reading of it.
+template <bool TURN>
+__global__ __launch_bounds__(TPB) void k_finalize_fin(Bufs B) {
+    const int d = blockIdx.x;
+    finalize_fin_domain(B, d);
+    if (TURN) {
+        __syncthreads();
+        if (threadIdx.x == 0) turn_domain(B, d);
+    }
+}
+
+template <bool TURN>
+__global__ __launch_bounds__(TPB) void k_phase_only_fin(Bufs B) {
+    const int d = blockIdx.x;
+    phase_only_fin_domain(B, d);
+    if (TURN) {
+        __syncthreads();
+        if (threadIdx.x == 0) turn_domain(B, d);
+    }
 }
 
 // Fixed-dt runs that keep every state (the adjoint's segment recompute):
