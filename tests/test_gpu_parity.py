@@ -519,11 +519,14 @@ def test_full_smoke_trajectory_vs_reference(pkg):
     assert dmu <= MU_TOL
 
 
-def test_batch_of_uneven_domains_matches_single_domains(pkg, O):
+@pytest.mark.parametrize("names", [["small_48x40", "window_alpha_32x40", "colony_256x128"],
+                                   ["small_48x40", "phase_only_48x40", "colony_256x128"]])
+def test_batch_of_uneven_domains_matches_single_domains(pkg, O, names):
     """Three domains of different sizes and physics in one context: each is
     bitwise equal to its own oracle trajectory in the context's blocked
-    order."""
-    names = ["small_48x40", "window_alpha_32x40", "colony_256x128"]
+    order.  With a phase-only domain in the context, the round ends in the
+    phase-only pass, whose finisher takes every domain's turn (the coupled
+    domains' too)."""
     gs = [load_golden(n) for n in names]
     steps = [12, 5, 8]
     say("batch of", names)
